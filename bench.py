@@ -1,0 +1,174 @@
+"""Benchmark: adjoint-Jacobian parameters per second at 24 qubits on one MI355X.
+
+Workload (one step): the 24-qubit strongly entangling circuit of 10 layers
+(Rot on every wire serialized as RZ, RY, RZ; then CNOT(i, (i+r) mod 24) with
+r = (layer mod 23) + 1) = 960 operations and 720 parameters, observables
+PauliZ on wires 0-7 plus one 32-term random Pauli Hamiltonian (9 outputs).
+A step is the full adjoint Jacobian as the device computes it: forward
+application of the circuit to |0...0>, the observables applied to the state,
+and the fused backward sweep. Inputs are resident in HBM before timing.
+
+Prints one JSON line. `roofline` refers to the dominant kernel (the fused
+adjoint segment): algorithmic bytes per launch / average launch duration,
+both measured with HIP events on the library's stream during the timed
+steps. `cpu_baseline` times the CPU oracle (the reference algorithm restated
+in C++/OpenMP) on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+for _p in (os.path.join(REPO, "pennylane-lightning-kokkos_amd"), os.path.join(REPO, "oracle"),
+           os.path.join(REPO, "tests")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def build_inputs(capi, n, ops, obs, dtype):
+    from helpers import to_capi
+
+    names, params, wires, inv, mats = ops
+    s = capi.State(n, dtype)
+    cops = capi.Ops(names, params, wires, inv, mats)
+    cobs = [to_capi(capi, o) for o in obs]
+    npar = sum(1 for p in params if p)
+    return s, cops, cobs, list(range(npar))
+
+
+def run_gpu(capi, n, ops, obs, dtype, steps, warmup, profile=True):
+    s, cops, cobs, tp = build_inputs(capi, n, ops, obs, dtype)
+    jac = None
+    for _ in range(warmup):
+        jac = capi.adjoint_jacobian(s, cobs, cops, tp, True)
+    s.synchronize()
+    if profile:
+        capi.profiling_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        jac = capi.adjoint_jacobian(s, cobs, cops, tp, True)
+    s.synchronize()
+    dt = time.perf_counter() - t0
+    stats = {}
+    if profile:
+        for k in capi.profiling_names():
+            stats[k] = capi.profiling_get(k)
+        capi.profiling_enable(False)
+    return jac, dt, stats, len(tp)
+
+
+def cpu_baseline(n, ops, obs, sample_params):
+    """Oracle (reference algorithm, C++/OpenMP) on the last `sample_params`
+    trainable parameters of the same circuit: forward pass, observable
+    application and the reverse sweep until those parameters are consumed."""
+    import oracle as O
+    from helpers import to_oracle
+
+    names, params, wires, inv, mats = ops
+    npar = sum(1 for p in params if p)
+    tp = list(range(npar - sample_params, npar))
+    psi = O.zero_state(n)
+    t0 = time.perf_counter()
+    O.adjoint_jacobian(psi, [to_oracle(o) for o in obs], O.Ops(names, params, wires, inv, mats), tp, True)
+    dt = time.perf_counter() - t0
+    return {"value": sample_params / dt, "unit": "params/s", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"oracle adjoint_jacobian on the same {n}-qubit circuit and 9 observables, "
+                       f"trainable = last {sample_params} of {npar} parameters (forward pass + "
+                       f"observable application + reverse sweep until consumed), one run, "
+                       f"{dt:.1f} s")}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--qubits", type=int, default=24)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-sample", type=int, default=24)
+    ap.add_argument("--c64-steps", type=int, default=3)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(sharding is design-only); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    from helpers import c3_workload
+    from pennylane_lightning_kokkos_amd import _capi as capi
+
+    capi.load()
+    n, ops, obs = c3_workload(n=args.qubits)
+    log(f"bench: {n} qubits, {len(ops[0])} ops, {len(obs)} observables; complex128 warmup")
+    jac, dt, stats, ntp = run_gpu(capi, n, ops, obs, np.complex128, args.steps, args.warmup)
+    ms = 1e3 * dt / args.steps
+    value = ntp * args.steps / dt
+    log(f"bench: complex128 {ms:.1f} ms/step, {value:.1f} params/s")
+
+    seg = stats.get("adjoint_segment", {"launches": 0, "total_ms": 0, "bytes": 0})
+    roof = None
+    if seg["launches"]:
+        avg_ms = seg["total_ms"] / seg["launches"]
+        avg_bytes = seg["bytes"] / seg["launches"]
+        achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "kernel": "seg_kernel (fused adjoint segment)",
+                "avg_launch_ms": round(avg_ms, 4), "avg_launch_bytes": avg_bytes,
+                "launches_per_step": seg["launches"] / args.steps}
+    breakdown = {k: {"ms_per_step": round(v["total_ms"] / args.steps, 3),
+                     "launches_per_step": v["launches"] / args.steps,
+                     "GBps": round(v["bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)}
+                 for k, v in stats.items()}
+
+    extra = {}
+    if args.c64_steps > 0:
+        log("bench: complex64 (FP64 reductions)")
+        jac64, dt64, _, _ = run_gpu(capi, n, ops, obs, np.complex64, args.c64_steps, 1, profile=False)
+        extra["complex64"] = {"params_per_s": round(ntp * args.c64_steps / dt64, 1),
+                              "ms_per_step": round(1e3 * dt64 / args.c64_steps, 2),
+                              "max_abs_diff_vs_complex128": float(np.max(np.abs(jac64 - jac)))}
+
+    cpu = None
+    if not args.no_cpu:
+        log(f"bench: CPU oracle baseline (last {args.cpu_sample} parameters)")
+        cpu = cpu_baseline(n, ops, obs, args.cpu_sample)
+        log(f"bench: CPU {cpu['value']:.2f} params/s on {cpu['cores']} threads")
+
+    out = {
+        "metric": "adjoint Jacobian params/s @24q (complex128, FP64 reductions)",
+        "value": round(value, 2),
+        "unit": "params/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded circuit parameters and Hamiltonian)",
+        "config": {"workload": f"adjoint Jacobian, {n}-qubit strongly entangling circuit, 10 layers, "
+                               f"960 ops, 720 params, 8 PauliZ + 32-term Pauli Hamiltonian",
+                   "qubits": n, "params": ntp, "observables": len(obs), "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernels": breakdown,
+        **extra,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

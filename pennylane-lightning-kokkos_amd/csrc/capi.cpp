@@ -1,0 +1,1148 @@
+// C ABI of the engine: states, gates, observables, measurements and the
+// adjoint Jacobian. See include/plk_amd.h for the contract of each entry
+// point and the reference member it replaces.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <set>
+#include <sstream>
+
+#include "plk_internal.hpp"
+
+namespace plk {
+
+namespace {
+
+thread_local std::string g_last_error;
+
+const cd kI{0.0, 1.0};
+
+Device &dev_of(const State &s) { return device(s.device); }
+
+// ---------------------------------------------------------------------------
+// Observables (ObservablesKokkos.hpp:19-517).
+// ---------------------------------------------------------------------------
+struct ObsNode {
+    enum Kind { Named, Hermitian, Tensor, Hamiltonian, Sparse } kind = Named;
+    std::string name;
+    std::vector<size_t> wires;
+    std::vector<cd> matrix;
+    std::vector<std::shared_ptr<const ObsNode>> terms;
+    std::vector<double> coeffs;
+    std::vector<double> data; // CSR values, interleaved complex
+    std::vector<int64_t> indices, indptr;
+    size_t rows = 0;
+};
+
+std::vector<size_t> obs_wires(const ObsNode &o) {
+    std::set<size_t> w;
+    if (o.kind == ObsNode::Tensor || o.kind == ObsNode::Hamiltonian) {
+        for (auto &t : o.terms) {
+            auto tw = obs_wires(*t);
+            w.insert(tw.begin(), tw.end());
+        }
+    } else {
+        w.insert(o.wires.begin(), o.wires.end());
+    }
+    return {w.begin(), w.end()};
+}
+
+constexpr size_t kMaxPauliTerms = 4096;
+
+uint64_t wire_bit(int n, size_t w) {
+    if (w >= static_cast<size_t>(n)) {
+        raise(PLK_ERR_INVALID, "observable wire " + std::to_string(w) + " out of range");
+    }
+    return uint64_t{1} << (n - 1 - static_cast<int>(w));
+}
+
+// Expands an observable into Pauli words when it is a sum of products of
+// Identity / PauliX / PauliY / PauliZ / Hadamard. Returns false otherwise.
+bool to_pauli(const ObsNode &o, int n, std::vector<PauliTerm> &out) {
+    switch (o.kind) {
+    case ObsNode::Named: {
+        if (o.name == "Identity") {
+            out = {PauliTerm{0, 0, 1.0}};
+            return true;
+        }
+        if (o.wires.size() != 1) {
+            return false;
+        }
+        const uint64_t b = wire_bit(n, o.wires[0]);
+        if (o.name == "PauliX") {
+            out = {PauliTerm{b, 0, 1.0}};
+        } else if (o.name == "PauliY") {
+            out = {PauliTerm{b, b, -kI}};
+        } else if (o.name == "PauliZ") {
+            out = {PauliTerm{0, b, 1.0}};
+        } else if (o.name == "Hadamard") {
+            out = {PauliTerm{b, 0, M_SQRT1_2}, PauliTerm{0, b, M_SQRT1_2}};
+        } else {
+            return false;
+        }
+        return true;
+    }
+    case ObsNode::Tensor: {
+        std::vector<PauliTerm> acc{PauliTerm{0, 0, 1.0}};
+        for (auto &f : o.terms) {
+            std::vector<PauliTerm> ft;
+            if (!to_pauli(*f, n, ft)) {
+                return false;
+            }
+            std::vector<PauliTerm> next;
+            if (acc.size() * ft.size() > kMaxPauliTerms) {
+                return false;
+            }
+            for (auto &a : acc) {
+                for (auto &b : ft) {
+                    // Factors act on disjoint wires, so words multiply bitwise.
+                    next.push_back({a.x | b.x, a.zy | b.zy, a.coeff * b.coeff});
+                }
+            }
+            acc.swap(next);
+        }
+        out = acc;
+        return true;
+    }
+    case ObsNode::Hamiltonian: {
+        std::vector<PauliTerm> acc;
+        for (size_t t = 0; t < o.terms.size(); t++) {
+            std::vector<PauliTerm> tt;
+            if (!to_pauli(*o.terms[t], n, tt)) {
+                return false;
+            }
+            for (auto &p : tt) {
+                p.coeff *= o.coeffs[t];
+                acc.push_back(p);
+            }
+            if (acc.size() > kMaxPauliTerms) {
+                return false;
+            }
+        }
+        out = acc;
+        return true;
+    }
+    default:
+        return false;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gate application.
+// ---------------------------------------------------------------------------
+struct PlanOp {
+    CompiledOp U;
+    bool trainable = false;
+    CompiledGen G;
+    int col = -1;
+    double coef = 0.0;
+};
+
+// Parity ops/generators on <= 2 wires become dense diagonals so they fuse.
+std::vector<cd> parity_diag(size_t m, cd even, cd odd) {
+    const size_t dim = size_t{1} << m;
+    std::vector<cd> M(dim * dim, 0.0);
+    for (size_t r = 0; r < dim; r++) {
+        M[r * dim + r] = (__builtin_popcountll(r) & 1) ? odd : even;
+    }
+    return M;
+}
+
+void densify(CompiledOp &op) {
+    if (op.kind == OpKind::Parity && op.wires.size() <= 2) {
+        op.mat = parity_diag(op.wires.size(), op.even, op.odd);
+        op.kind = OpKind::Dense;
+    }
+}
+
+void densify(CompiledGen &g) {
+    if (g.kind == OpKind::Parity && g.wires.size() <= 2) {
+        g.mat = parity_diag(g.wires.size(), 1.0, -1.0);
+        g.kind = OpKind::Dense;
+    }
+}
+
+bool fusable(const PlanOp &p) {
+    if (p.U.kind != OpKind::Dense || p.U.wires.size() > 2) {
+        return false;
+    }
+    if (p.trainable && (p.G.kind != OpKind::Dense || p.G.wires.size() > 2)) {
+        return false;
+    }
+    return true;
+}
+
+struct Segment {
+    std::vector<size_t> wires; // global wires of the segment
+    std::vector<PlanOp> ops;
+    int nslots = 0;
+};
+
+struct Step {
+    bool fused = false;
+    Segment seg;
+    PlanOp single;
+};
+
+constexpr size_t kSegMaxOps = 64;
+
+// Greedy fusion of consecutive operations into register segments.
+std::vector<Step> plan_steps(std::vector<PlanOp> &&ops) {
+    std::vector<Step> steps;
+    Segment cur;
+    auto flush = [&]() {
+        if (!cur.ops.empty()) {
+            Step s;
+            s.fused = true;
+            s.seg = std::move(cur);
+            steps.push_back(std::move(s));
+        }
+        cur = Segment{};
+    };
+    for (auto &p : ops) {
+        if (p.U.kind == OpKind::Identity && !p.trainable) {
+            continue;
+        }
+        densify(p.U);
+        if (p.trainable) {
+            densify(p.G);
+        }
+        if (!fusable(p)) {
+            flush();
+            Step s;
+            s.fused = false;
+            s.single = std::move(p);
+            steps.push_back(std::move(s));
+            continue;
+        }
+        std::set<size_t> w(cur.wires.begin(), cur.wires.end());
+        w.insert(p.U.wires.begin(), p.U.wires.end());
+        if (w.size() > static_cast<size_t>(SEG_KMAX) || cur.ops.size() >= kSegMaxOps ||
+            (p.trainable && cur.nslots >= SEG_PMAX)) {
+            flush();
+            w = std::set<size_t>(p.U.wires.begin(), p.U.wires.end());
+        }
+        cur.wires.assign(w.begin(), w.end());
+        if (p.trainable) {
+            cur.nslots++;
+        }
+        cur.ops.push_back(std::move(p));
+    }
+    flush();
+    return steps;
+}
+
+struct SegMeta {
+    SegLaunch L{};
+    size_t first = 0; // index of the first SegOp in the uploaded table
+    int cols[SEG_PMAX] = {0};
+    double coefs[SEG_PMAX] = {0};
+};
+
+template <typename T>
+void put_mat(T *dst, const std::vector<cd> &M, bool swap_bits) {
+    const size_t dim = M.size() == 16 ? 4 : 2;
+    auto sw = [&](size_t r) { return swap_bits ? (((r & 1) << 1) | ((r >> 1) & 1)) : r; };
+    for (size_t r = 0; r < dim; r++) {
+        for (size_t c = 0; c < dim; c++) {
+            const cd v = M[sw(r) * dim + sw(c)];
+            dst[2 * (r * dim + c)] = static_cast<T>(v.real());
+            dst[2 * (r * dim + c) + 1] = static_cast<T>(v.imag());
+        }
+    }
+}
+
+template <typename T>
+SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
+    SegMeta meta;
+    SegLaunch &L = meta.L;
+    L.n = n;
+    L.K = static_cast<int>(seg.wires.size());
+    std::vector<std::pair<int, size_t>> bw; // (bit, wire)
+    for (auto w : seg.wires) {
+        bw.push_back({n - 1 - static_cast<int>(w), w});
+    }
+    std::sort(bw.begin(), bw.end());
+    std::map<size_t, int> pos;
+    for (int q = 0; q < L.K; q++) {
+        L.bits[q] = bw[q].first;
+        pos[bw[q].second] = q;
+    }
+    meta.first = table.size();
+    int slot = 0;
+    for (const auto &p : seg.ops) {
+        SegOp<T> d{};
+        d.slot = -1;
+        const auto &w = p.U.wires;
+        bool swapb = false;
+        if (w.size() == 1) {
+            d.cls = pos.at(w[0]);
+        } else {
+            int qa = pos.at(w[0]), qb = pos.at(w[1]);
+            if (qa < qb) {
+                std::swap(qa, qb);
+                swapb = true;
+            }
+            if (L.K == 2) {
+                d.cls = 2;
+            } else {
+                d.cls = (qa == 1) ? 3 : (qb == 0 ? 4 : 5);
+            }
+        }
+        put_mat(d.U, p.U.mat, swapb);
+        if (p.trainable) {
+            if (p.G.wires != p.U.wires) {
+                raise(PLK_ERR_INVALID, "generator wires differ from gate wires");
+            }
+            put_mat(d.G, p.G.mat, swapb);
+            d.slot = slot;
+            meta.cols[slot] = p.col;
+            meta.coefs[slot] = p.coef;
+            slot++;
+        }
+        table.push_back(d);
+    }
+    L.nops = static_cast<int>(seg.ops.size());
+    L.nslots = slot;
+    return meta;
+}
+
+// Device table of segment descriptors for a whole plan (one upload).
+struct SegTable {
+    void *d = nullptr;
+    size_t elem = 0;
+    ~SegTable() {
+        if (d) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(d);
+        }
+    }
+};
+
+template <typename T>
+std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable &tab) {
+    std::vector<SegOp<T>> table;
+    std::vector<SegMeta> metas;
+    for (const auto &s : steps) {
+        if (s.fused) {
+            metas.push_back(build_segment<T>(s.seg, n, table));
+        } else {
+            metas.push_back(SegMeta{});
+        }
+    }
+    tab.elem = sizeof(SegOp<T>);
+    if (!table.empty()) {
+        PLK_HIP(hipMalloc(&tab.d, table.size() * sizeof(SegOp<T>)));
+        PLK_HIP(hipMemcpy(tab.d, table.data(), table.size() * sizeof(SegOp<T>),
+                          hipMemcpyHostToDevice));
+    }
+    return metas;
+}
+
+std::vector<SegMeta> upload_plan_any(const std::vector<Step> &steps, const State &s,
+                                     SegTable &tab) {
+    return s.dtype == PLK_C64 ? upload_plan<float>(steps, s.n, tab)
+                              : upload_plan<double>(steps, s.n, tab);
+}
+
+void apply_single(Device &dev, const State &s, const CompiledOp &op);
+
+// Applies a forward plan (no observables) to s.
+void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
+    SegTable tab;
+    auto metas = upload_plan_any(steps, s, tab);
+    for (size_t i = 0; i < steps.size(); i++) {
+        if (steps[i].fused) {
+            SegLaunch L = metas[i].L;
+            L.write_phi = 1;
+            L.nj = 0;
+            L.nslots = 0;
+            const void *ops = static_cast<const char *>(tab.d) + metas[i].first * tab.elem;
+            launch_segment(dev, s, nullptr, ops, L);
+        } else {
+            apply_single(dev, s, steps[i].single.U);
+        }
+    }
+}
+
+void apply_single(Device &dev, const State &s, const CompiledOp &op) {
+    switch (op.kind) {
+    case OpKind::Identity:
+        return;
+    case OpKind::Parity:
+        launch_parity(dev, s, op.wires, op.even, op.odd);
+        return;
+    case OpKind::Dense:
+        if (op.wires.size() <= 2) {
+            PlanOp p;
+            p.U = op;
+            std::vector<PlanOp> v;
+            v.push_back(std::move(p));
+            run_forward(dev, s, plan_steps(std::move(v)));
+        } else {
+            launch_dense(dev, s, op);
+        }
+        return;
+    }
+}
+
+void apply_generator_dev(Device &dev, const State &s, const CompiledGen &g) {
+    if (g.kind == OpKind::Parity) {
+        launch_parity(dev, s, g.wires, 1.0, -1.0);
+    } else {
+        launch_dense_matrix(dev, s, g.wires, g.mat);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Observable application and expectation values.
+// ---------------------------------------------------------------------------
+void apply_obs_general(Device &dev, const State &s, const ObsNode &o);
+
+void apply_obs_into(Device &dev, const State &out, const State &in, const ObsNode &o) {
+    std::vector<PauliTerm> terms;
+    if (to_pauli(o, in.n, terms)) {
+        launch_pauli_apply(dev, out, in, terms);
+        return;
+    }
+    launch_copy(dev, out, in);
+    apply_obs_general(dev, out, o);
+}
+
+void apply_obs_general(Device &dev, const State &s, const ObsNode &o) {
+    switch (o.kind) {
+    case ObsNode::Named:
+        apply_single(dev, s, compile_op(o.name, o.wires, false, {}, {}, s.n));
+        return;
+    case ObsNode::Hermitian: {
+        CompiledOp op;
+        op.kind = OpKind::Dense;
+        op.wires = o.wires;
+        op.mat = o.matrix;
+        apply_single(dev, s, op);
+        return;
+    }
+    case ObsNode::Tensor:
+        for (auto &t : o.terms) {
+            apply_obs_general(dev, s, *t);
+        }
+        return;
+    case ObsNode::Hamiltonian: {
+        TmpState buf(s.n, s.dtype, s.device);
+        TmpState tmp(s.n, s.dtype, s.device);
+        launch_fill_zero(dev, buf.s);
+        for (size_t t = 0; t < o.terms.size(); t++) {
+            apply_obs_into(dev, tmp.s, s, *o.terms[t]);
+            launch_axpy(dev, buf.s, o.coeffs[t], tmp.s);
+        }
+        launch_copy(dev, s, buf.s);
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+        return;
+    }
+    case ObsNode::Sparse: {
+        if (o.wires.size() != static_cast<size_t>(s.n)) {
+            raise(PLK_ERR_INVALID, "SparseH wire count does not match state-vector size");
+        }
+        TmpState tmp(s.n, s.dtype, s.device);
+        csr_spmv(dev, tmp.s, s, o.data.data(), o.indices.data(), o.indptr.data(),
+                 o.indices.size(), o.rows);
+        launch_copy(dev, s, tmp.s);
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+        return;
+    }
+    }
+}
+
+double expval_obs(Device &dev, const State &s, const ObsNode &o) {
+    std::vector<PauliTerm> terms;
+    if (to_pauli(o, s.n, terms)) {
+        return pauli_expval(dev, s, terms);
+    }
+    TmpState tmp(s.n, s.dtype, s.device);
+    launch_copy(dev, tmp.s, s);
+    apply_obs_general(dev, tmp.s, o);
+    const State *xs[1] = {&s};
+    double ri[2];
+    inner_products(dev, xs, 1, tmp.s, ri);
+    return ri[0];
+}
+
+// ---------------------------------------------------------------------------
+// Operation list (AdjointDiffKokkos.hpp:17-173).
+// ---------------------------------------------------------------------------
+struct OpsData {
+    std::vector<std::string> names;
+    std::vector<std::vector<double>> params;
+    std::vector<std::vector<size_t>> wires;
+    std::vector<bool> inverses;
+    std::vector<std::vector<cd>> matrices;
+    size_t num_par_ops() const {
+        size_t c = 0;
+        for (auto &p : params) {
+            c += p.empty() ? 0 : 1;
+        }
+        return c;
+    }
+};
+
+// Forward plan of the whole op list (applyOperations, AdjointDiffKokkos.hpp:213-226).
+std::vector<Step> forward_plan(const OpsData &ops, int n) {
+    std::vector<PlanOp> v;
+    for (size_t i = 0; i < ops.names.size(); i++) {
+        PlanOp p;
+        p.U = compile_op(ops.names[i], ops.wires[i], ops.inverses[i], ops.params[i],
+                         ops.matrices[i], n);
+        v.push_back(std::move(p));
+    }
+    return plan_steps(std::move(v));
+}
+
+// ---------------------------------------------------------------------------
+// Adjoint Jacobian (AdjointDiffKokkos.hpp:404-478).
+//
+// The reverse walk over the operations, the trainable-parameter bookkeeping
+// and the early stop follow the reference line by line; the per-operation
+// work (mu copy, U^dagger on lambda, generator on mu, n_obs inner products,
+// U^dagger on every H lambda) is replaced by fused segments.
+// ---------------------------------------------------------------------------
+void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
+                      const OpsData &ops, const std::vector<size_t> &tp, bool apply_ops,
+                      double *jac) {
+    if (tp.empty()) {
+        raise(PLK_ERR_INVALID, "No trainable parameters provided.");
+    }
+    const size_t n_obs = obs.size();
+    const size_t n_tp = tp.size();
+    std::fill(jac, jac + n_obs * n_tp, 0.0);
+    Device &dev = dev_of(ref);
+    const int n = ref.n;
+
+    // Reverse plan first: it validates the op list before any device work.
+    std::vector<PlanOp> rev;
+    {
+        size_t trainable_number = n_tp - 1;
+        size_t current_param_idx = ops.num_par_ops() - 1;
+        auto tp_it = tp.rbegin();
+        for (long op_idx = static_cast<long>(ops.names.size()) - 1; op_idx >= 0; op_idx--) {
+            const size_t o = static_cast<size_t>(op_idx);
+            if (ops.params[o].size() > 1) {
+                raise(PLK_ERR_INVALID,
+                      "The operation is not supported using the adjoint "
+                      "differentiation method");
+            }
+            if (ops.names[o] == "StatePrep" || ops.names[o] == "BasisState") {
+                continue;
+            }
+            if (tp_it == tp.rend()) {
+                break;
+            }
+            PlanOp p;
+            p.U = compile_op(ops.names[o], ops.wires[o], !ops.inverses[o], ops.params[o],
+                             ops.matrices[o], n);
+            if (!ops.params[o].empty()) {
+                if (current_param_idx == *tp_it) {
+                    p.trainable = true;
+                    p.G = compile_generator(ops.names[o], ops.wires[o], n);
+                    p.coef = -2.0 * p.G.scale * (ops.inverses[o] ? -1.0 : 1.0);
+                    p.col = static_cast<int>(trainable_number);
+                    trainable_number--;
+                    ++tp_it;
+                }
+                current_param_idx--;
+            }
+            rev.push_back(std::move(p));
+        }
+    }
+    if (n_obs == 0) {
+        return;
+    }
+    const std::vector<Step> steps = plan_steps(std::move(rev));
+
+    TmpState lambda(n, ref.dtype, ref.device);
+    launch_copy(dev, lambda.s, ref);
+    if (apply_ops) {
+        run_forward(dev, lambda.s, forward_plan(ops, n));
+    }
+    std::vector<std::unique_ptr<TmpState>> H;
+    for (size_t j = 0; j < n_obs; j++) {
+        H.push_back(std::make_unique<TmpState>(n, ref.dtype, ref.device));
+        apply_obs_into(dev, H.back()->s, lambda.s, *obs[j]);
+    }
+    std::vector<void *> hptr(n_obs);
+    std::vector<const State *> hstate(n_obs);
+    for (size_t j = 0; j < n_obs; j++) {
+        hptr[j] = H[j]->s.d;
+        hstate[j] = &H[j]->s;
+    }
+
+    double *d_jac = nullptr;
+    PLK_HIP(hipMalloc(&d_jac, sizeof(double) * n_obs * n_tp));
+    PLK_HIP(hipMemsetAsync(d_jac, 0, sizeof(double) * n_obs * n_tp, dev.stream));
+    std::vector<std::pair<size_t, double>> host_entries; // (flat index, value)
+
+    SegTable tab;
+    const auto metas = upload_plan_any(steps, lambda.s, tab);
+    std::unique_ptr<TmpState> mu;
+    const int kmax_nj = SEG_NJMAX;
+    for (size_t i = 0; i < steps.size(); i++) {
+        const Step &st = steps[i];
+        if (st.fused) {
+            const SegMeta &m = metas[i];
+            const void *d_ops = static_cast<const char *>(tab.d) + m.first * tab.elem;
+            const int njmax = std::min(kmax_nj, seg_max_nj(ref.dtype, m.L.K));
+            size_t j0 = 0;
+            while (j0 < n_obs) {
+                const size_t left = n_obs - j0;
+                int nj = 1;
+                for (int c : {8, 4, 2, 1}) {
+                    if (c <= njmax && static_cast<size_t>(c) <= left) {
+                        nj = c;
+                        break;
+                    }
+                }
+                SegLaunch L = m.L;
+                L.nj = nj;
+                L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+                const int grid = launch_segment(dev, lambda.s, hptr.data() + j0, d_ops, L);
+                if (L.nslots > 0) {
+                    launch_segment_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0), m.cols,
+                                          m.coefs, d_jac, static_cast<int>(n_tp));
+                }
+                j0 += nj;
+            }
+        } else {
+            const PlanOp &p = st.single;
+            if (p.trainable) {
+                if (!mu) {
+                    mu = std::make_unique<TmpState>(n, ref.dtype, ref.device);
+                }
+                launch_copy(dev, mu->s, lambda.s);
+            }
+            apply_single(dev, lambda.s, p.U);
+            if (p.trainable) {
+                apply_generator_dev(dev, mu->s, p.G);
+                std::vector<double> ri(2 * n_obs);
+                inner_products(dev, hstate.data(), static_cast<int>(n_obs), mu->s, ri.data());
+                for (size_t j = 0; j < n_obs; j++) {
+                    host_entries.push_back({j * n_tp + static_cast<size_t>(p.col),
+                                            p.coef * ri[2 * j + 1]});
+                }
+            }
+            for (size_t j = 0; j < n_obs; j++) {
+                apply_single(dev, H[j]->s, p.U);
+            }
+        }
+    }
+    PLK_HIP(hipMemcpyAsync(jac, d_jac, sizeof(double) * n_obs * n_tp, hipMemcpyDeviceToHost,
+                           dev.stream));
+    PLK_HIP(hipStreamSynchronize(dev.stream));
+    PLK_HIP(hipFree(d_jac));
+    for (auto &e : host_entries) {
+        jac[e.first] = e.second;
+    }
+}
+
+std::vector<size_t> to_vec(const size_t *w, size_t n) {
+    return std::vector<size_t>(w, w + n);
+}
+
+std::vector<cd> to_cvec(const double *m, size_t len) {
+    std::vector<cd> v(len);
+    for (size_t i = 0; i < len; i++) {
+        v[i] = cd{m[2 * i], m[2 * i + 1]};
+    }
+    return v;
+}
+
+State &S(plk_state *p) {
+    if (!p) {
+        raise(PLK_ERR_INVALID, "null state handle");
+    }
+    return p->s;
+}
+
+const State &S(const plk_state *p) {
+    if (!p) {
+        raise(PLK_ERR_INVALID, "null state handle");
+    }
+    return p->s;
+}
+
+} // namespace
+
+} // namespace plk
+
+struct plk_obs {
+    std::shared_ptr<const plk::ObsNode> node;
+};
+
+struct plk_ops {
+    plk::OpsData data;
+};
+
+using namespace plk;
+
+#define PLK_TRY(body)                                                          \
+    try {                                                                      \
+        body;                                                                  \
+        g_last_error.clear();                                                  \
+        return PLK_OK;                                                         \
+    } catch (const plk::Error &e) {                                            \
+        g_last_error = e.msg;                                                  \
+        return e.code;                                                         \
+    } catch (const std::bad_alloc &) {                                         \
+        g_last_error = "host allocation failed";                               \
+        return PLK_ERR_OOM;                                                    \
+    } catch (const std::exception &e) {                                        \
+        g_last_error = e.what();                                               \
+        return PLK_ERR_INVALID;                                                \
+    }
+
+static const ObsNode &O(const plk_obs *o) {
+    if (!o || !o->node) {
+        raise(PLK_ERR_INVALID, "null observable handle");
+    }
+    return *o->node;
+}
+
+extern "C" {
+
+const char *plk_last_error(void) { return g_last_error.c_str(); }
+
+int plk_device_count(int *out) {
+    PLK_TRY({
+        int c = 0;
+        PLK_HIP(hipGetDeviceCount(&c));
+        *out = c;
+    })
+}
+
+int plk_print_configuration(char *buf, size_t len) {
+    PLK_TRY({
+        std::ostringstream os;
+        int count = 0;
+        PLK_HIP(hipGetDeviceCount(&count));
+        os << "plk_amd state-vector engine\n";
+        os << "  HIP runtime devices: " << count << "\n";
+        for (int d = 0; d < count; d++) {
+            hipDeviceProp_t p;
+            PLK_HIP(hipGetDeviceProperties(&p, d));
+            os << "  device " << d << ": " << p.name << " arch=" << p.gcnArchName
+               << " CUs=" << p.multiProcessorCount
+               << " HBM=" << (p.totalGlobalMem >> 20) << " MiB"
+               << " LDS/block=" << p.sharedMemPerBlock << " B"
+               << " wavefront=" << p.warpSize << "\n";
+        }
+        os << "  build: gfx950 HIP kernels, complex64/complex128 states, FP64 reductions\n";
+        os << "  fused segments: up to " << SEG_KMAX << " wires, " << SEG_PMAX
+           << " trainable ops and " << SEG_NJMAX << " observable states per HBM pass\n";
+        const std::string s = os.str();
+        if (len > 0) {
+            std::strncpy(buf, s.c_str(), len - 1);
+            buf[len - 1] = '\0';
+        }
+    })
+}
+
+int plk_state_create(int num_qubits, int dtype, int device_id, plk_state **out) {
+    PLK_TRY({
+        if (num_qubits < 0 || num_qubits > 40) {
+            raise(PLK_ERR_INVALID, "number of qubits out of range");
+        }
+        if (dtype != PLK_C64 && dtype != PLK_C128) {
+            raise(PLK_ERR_INVALID, "dtype must be PLK_C64 or PLK_C128");
+        }
+        Device &dev = device(device_id);
+        auto h = std::make_unique<plk_state>();
+        h->s.n = num_qubits;
+        h->s.dtype = dtype;
+        h->s.len = size_t{1} << num_qubits;
+        h->s.device = device_id;
+        PLK_HIP(hipMalloc(&h->s.d, h->s.bytes()));
+        launch_set_basis(dev, h->s, 0);
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+        *out = h.release();
+    })
+}
+
+int plk_state_destroy(plk_state *sv) {
+    PLK_TRY({
+        if (sv) {
+            Device &dev = device(sv->s.device);
+            PLK_HIP(hipStreamSynchronize(dev.stream));
+            PLK_HIP(hipFree(sv->s.d));
+            delete sv;
+        }
+    })
+}
+
+int plk_state_num_qubits(const plk_state *sv, int *out) { PLK_TRY({ *out = S(sv).n; }) }
+
+int plk_state_length(const plk_state *sv, size_t *out) { PLK_TRY({ *out = S(sv).len; }) }
+
+int plk_state_dtype(const plk_state *sv, int *out) { PLK_TRY({ *out = S(sv).dtype; }) }
+
+int plk_state_device_ptr(plk_state *sv, void **out) { PLK_TRY({ *out = S(sv).d; }) }
+
+int plk_state_upload(plk_state *sv, const void *host, size_t length) {
+    PLK_TRY({
+        State &s = S(sv);
+        if (length != s.len) {
+            raise(PLK_ERR_INVALID, "HostToDevice: length " + std::to_string(length) +
+                                       " does not match the state length " +
+                                       std::to_string(s.len));
+        }
+        Device &dev = dev_of(s);
+        PLK_HIP(hipMemcpyAsync(s.d, host, s.bytes(), hipMemcpyHostToDevice, dev.stream));
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+    })
+}
+
+int plk_state_download(const plk_state *sv, void *host, size_t length) {
+    PLK_TRY({
+        const State &s = S(sv);
+        if (length != s.len) {
+            raise(PLK_ERR_INVALID, "DeviceToHost: length " + std::to_string(length) +
+                                       " does not match the state length " +
+                                       std::to_string(s.len));
+        }
+        Device &dev = dev_of(s);
+        PLK_HIP(hipMemcpyAsync(host, s.d, s.bytes(), hipMemcpyDeviceToHost, dev.stream));
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+    })
+}
+
+int plk_state_reset(plk_state *sv) {
+    PLK_TRY({
+        State &s = S(sv);
+        launch_set_basis(dev_of(s), s, 0);
+    })
+}
+
+int plk_state_set_basis(plk_state *sv, size_t index) {
+    PLK_TRY({
+        State &s = S(sv);
+        if (index >= s.len) {
+            raise(PLK_ERR_INVALID, "basis-state index out of range");
+        }
+        launch_set_basis(dev_of(s), s, index);
+    })
+}
+
+int plk_state_set_values(plk_state *sv, const size_t *indices, const void *values,
+                         size_t count) {
+    PLK_TRY({
+        State &s = S(sv);
+        Device &dev = dev_of(s);
+        for (size_t i = 0; i < count; i++) {
+            if (indices[i] >= s.len) {
+                raise(PLK_ERR_INVALID, "setStateVector: index out of range");
+            }
+        }
+        launch_fill_zero(dev, s);
+        if (count) {
+            void *blob = nullptr;
+            const size_t ib = count * sizeof(size_t);
+            const size_t vo = (ib + 255) / 256 * 256;
+            PLK_HIP(hipMalloc(&blob, vo + count * s.elem_bytes()));
+            PLK_HIP(hipMemcpy(blob, indices, ib, hipMemcpyHostToDevice));
+            PLK_HIP(hipMemcpy(static_cast<char *>(blob) + vo, values, count * s.elem_bytes(),
+                              hipMemcpyHostToDevice));
+            launch_scatter(dev, s, static_cast<const size_t *>(blob),
+                           static_cast<char *>(blob) + vo, count);
+            PLK_HIP(hipStreamSynchronize(dev.stream));
+            PLK_HIP(hipFree(blob));
+        }
+    })
+}
+
+int plk_state_copy(plk_state *dst, const plk_state *src) {
+    PLK_TRY({
+        State &d = S(dst);
+        const State &s = S(src);
+        if (d.device != s.device) {
+            raise(PLK_ERR_INVALID, "state copy across devices is not supported");
+        }
+        launch_copy(dev_of(d), d, s);
+    })
+}
+
+int plk_state_synchronize(const plk_state *sv) {
+    PLK_TRY({ PLK_HIP(hipStreamSynchronize(dev_of(S(sv)).stream)); })
+}
+
+int plk_apply(plk_state *sv, const char *name, const size_t *wires, size_t num_wires,
+              int inverse, const double *params, size_t num_params) {
+    PLK_TRY({
+        State &s = S(sv);
+        const std::string nm(name ? name : "");
+        if (!is_named_gate(nm)) {
+            raise(PLK_ERR_UNSUPPORTED, "operation " + nm + " has no named kernel");
+        }
+        std::vector<double> p(params, params + num_params);
+        apply_single(dev_of(s), s,
+                     compile_op(nm, to_vec(wires, num_wires), inverse != 0, p, {}, s.n));
+    })
+}
+
+int plk_apply_matrix(plk_state *sv, const double *matrix, const size_t *wires,
+                     size_t num_wires, int inverse) {
+    PLK_TRY({
+        State &s = S(sv);
+        const size_t dim = size_t{1} << num_wires;
+        apply_single(dev_of(s), s,
+                     compile_op("Matrix", to_vec(wires, num_wires), inverse != 0, {},
+                                to_cvec(matrix, dim * dim), s.n));
+    })
+}
+
+int plk_apply_ops(plk_state *sv, const plk_ops *ops) {
+    PLK_TRY({
+        State &s = S(sv);
+        if (!ops) {
+            raise(PLK_ERR_INVALID, "null ops handle");
+        }
+        Device &dev = dev_of(s);
+        run_forward(dev, s, forward_plan(ops->data, s.n));
+    })
+}
+
+int plk_apply_generator(plk_state *sv, const char *name, const size_t *wires,
+                        size_t num_wires, int adjoint, double *scale) {
+    PLK_TRY({
+        (void)adjoint;
+        State &s = S(sv);
+        CompiledGen g = compile_generator(name ? name : "", to_vec(wires, num_wires), s.n);
+        apply_generator_dev(dev_of(s), s, g);
+        *scale = g.scale;
+    })
+}
+
+int plk_expval_named(const plk_state *sv, const char *name, const size_t *wires,
+                     size_t num_wires, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        ObsNode o;
+        o.kind = ObsNode::Named;
+        o.name = name ? name : "";
+        o.wires = to_vec(wires, num_wires);
+        std::vector<PauliTerm> terms;
+        if (!to_pauli(o, s.n, terms)) {
+            raise(PLK_ERR_INVALID, "named observable " + o.name + " is not supported");
+        }
+        *out = pauli_expval(dev_of(s), s, terms);
+    })
+}
+
+int plk_expval_matrix(const plk_state *sv, const double *matrix, const size_t *wires,
+                      size_t num_wires, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        ObsNode o;
+        o.kind = ObsNode::Hermitian;
+        o.wires = to_vec(wires, num_wires);
+        const size_t dim = size_t{1} << num_wires;
+        o.matrix = to_cvec(matrix, dim * dim);
+        *out = expval_obs(dev_of(s), s, o);
+    })
+}
+
+int plk_expval_csr(const plk_state *sv, const double *data, const int64_t *indices,
+                   const int64_t *indptr, size_t nnz, size_t num_rows, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        *out = csr_expval(dev_of(s), s, data, indices, indptr, nnz, num_rows);
+    })
+}
+
+int plk_expval_obs(const plk_state *sv, const plk_obs *ob, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        *out = expval_obs(dev_of(s), s, O(ob));
+    })
+}
+
+int plk_inner_product(const plk_state *x, const plk_state *y, double *re, double *im) {
+    PLK_TRY({
+        const State &a = S(x);
+        const State &b = S(y);
+        if (a.len != b.len || a.dtype != b.dtype) {
+            raise(PLK_ERR_INVALID, "inner product of states with different shapes");
+        }
+        const State *xs[1] = {&a};
+        double ri[2];
+        inner_products(dev_of(b), xs, 1, b, ri);
+        *re = ri[0];
+        *im = ri[1];
+    })
+}
+
+int plk_obs_named(const char *name, const size_t *wires, size_t num_wires, plk_obs **out) {
+    PLK_TRY({
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Named;
+        n->name = name ? name : "";
+        n->wires = to_vec(wires, num_wires);
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_hermitian(const double *matrix, const size_t *wires, size_t num_wires,
+                      plk_obs **out) {
+    PLK_TRY({
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Hermitian;
+        n->wires = to_vec(wires, num_wires);
+        const size_t dim = size_t{1} << num_wires;
+        n->matrix = to_cvec(matrix, dim * dim);
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_tensor(const plk_obs *const *factors, size_t num_factors, plk_obs **out) {
+    PLK_TRY({
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Tensor;
+        std::set<size_t> seen;
+        for (size_t i = 0; i < num_factors; i++) {
+            const ObsNode &f = O(factors[i]);
+            for (auto w : obs_wires(f)) {
+                if (!seen.insert(w).second) {
+                    raise(PLK_ERR_INVALID, "All wires in observables must be disjoint.");
+                }
+            }
+            n->terms.push_back(factors[i]->node);
+        }
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_hamiltonian(const double *coeffs, const plk_obs *const *terms, size_t num_terms,
+                        plk_obs **out) {
+    PLK_TRY({
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Hamiltonian;
+        for (size_t i = 0; i < num_terms; i++) {
+            O(terms[i]);
+            n->terms.push_back(terms[i]->node);
+            n->coeffs.push_back(coeffs[i]);
+        }
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_sparse(const double *data, const int64_t *indices, const int64_t *indptr,
+                   size_t nnz, size_t num_rows, const size_t *wires, size_t num_wires,
+                   plk_obs **out) {
+    PLK_TRY({
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Sparse;
+        n->data.assign(data, data + 2 * nnz);
+        n->indices.assign(indices, indices + nnz);
+        n->indptr.assign(indptr, indptr + num_rows + 1);
+        n->rows = num_rows;
+        n->wires = to_vec(wires, num_wires);
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_destroy(plk_obs *ob) { PLK_TRY({ delete ob; }) }
+
+int plk_obs_wires(const plk_obs *ob, size_t *wires, size_t cap, size_t *count) {
+    PLK_TRY({
+        auto w = obs_wires(O(ob));
+        *count = w.size();
+        for (size_t i = 0; i < w.size() && i < cap; i++) {
+            wires[i] = w[i];
+        }
+    })
+}
+
+int plk_obs_apply(plk_state *sv, const plk_obs *ob) {
+    PLK_TRY({
+        State &s = S(sv);
+        Device &dev = dev_of(s);
+        std::vector<PauliTerm> terms;
+        if (to_pauli(O(ob), s.n, terms)) {
+            TmpState tmp(s.n, s.dtype, s.device);
+            launch_pauli_apply(dev, tmp.s, s, terms);
+            launch_copy(dev, s, tmp.s);
+            PLK_HIP(hipStreamSynchronize(dev.stream));
+        } else {
+            apply_obs_general(dev, s, O(ob));
+        }
+    })
+}
+
+int plk_ops_create(size_t num_ops, const char *const *names, const double *params,
+                   const size_t *param_offsets, const size_t *wires,
+                   const size_t *wire_offsets, const int *inverses, const double *matrices,
+                   const size_t *matrix_offsets, plk_ops **out) {
+    PLK_TRY({
+        auto h = std::make_unique<plk_ops>();
+        OpsData &d = h->data;
+        for (size_t i = 0; i < num_ops; i++) {
+            d.names.emplace_back(names[i]);
+            d.params.emplace_back(params + param_offsets[i], params + param_offsets[i + 1]);
+            d.wires.emplace_back(wires + wire_offsets[i], wires + wire_offsets[i + 1]);
+            d.inverses.push_back(inverses[i] != 0);
+            const size_t ml = matrix_offsets[i + 1] - matrix_offsets[i];
+            d.matrices.push_back(ml ? to_cvec(matrices + 2 * matrix_offsets[i], ml)
+                                    : std::vector<cd>{});
+        }
+        *out = h.release();
+    })
+}
+
+int plk_ops_destroy(plk_ops *ops) { PLK_TRY({ delete ops; }) }
+
+int plk_ops_num_par_ops(const plk_ops *ops, size_t *out) {
+    PLK_TRY({
+        if (!ops) {
+            raise(PLK_ERR_INVALID, "null ops handle");
+        }
+        *out = ops->data.num_par_ops();
+    })
+}
+
+int plk_adjoint_jacobian(const plk_state *sv, const plk_obs *const *obs, size_t num_obs,
+                         const plk_ops *ops, const size_t *trainable, size_t num_trainable,
+                         int apply_operations, double *jac) {
+    PLK_TRY({
+        const State &s = S(sv);
+        if (!ops) {
+            raise(PLK_ERR_INVALID, "null ops handle");
+        }
+        std::vector<const ObsNode *> ob;
+        for (size_t j = 0; j < num_obs; j++) {
+            ob.push_back(&O(obs[j]));
+        }
+        std::vector<size_t> tp(trainable, trainable + num_trainable);
+        adjoint_jacobian(s, ob, ops->data, tp, apply_operations != 0, jac);
+    })
+}
+
+int plk_profiling_enable(int on) { PLK_TRY({ profiling_enable_impl(on != 0); }) }
+
+int plk_profiling_get(const char *kernel, double *launches, double *total_ms,
+                      double *bytes) {
+    PLK_TRY({
+        KernelStats k = profiling_get_impl(kernel ? kernel : "");
+        *launches = k.launches;
+        *total_ms = k.ms;
+        *bytes = k.bytes;
+    })
+}
+
+int plk_profiling_names(char *buf, size_t len) {
+    PLK_TRY({
+        const std::string s = profiling_names_impl();
+        if (len > 0) {
+            std::strncpy(buf, s.c_str(), len - 1);
+            buf[len - 1] = '\0';
+        }
+    })
+}
+
+} // extern "C"

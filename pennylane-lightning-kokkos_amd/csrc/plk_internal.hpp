@@ -1,0 +1,216 @@
+// Internal declarations of the gfx950 state-vector engine (host and device).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <cstdint>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "plk_amd.h"
+
+namespace plk {
+
+using cd = std::complex<double>;
+
+// ---------------------------------------------------------------------------
+// Errors: every C entry point converts an Error into a status code plus a
+// thread-local message (the reference's PL_ABORT -> LightningException path,
+// util/Error.hpp:27-142).
+// ---------------------------------------------------------------------------
+struct Error {
+    int code;
+    std::string msg;
+};
+
+[[noreturn]] void raise(int code, const std::string &msg);
+void check_hip(hipError_t e, const char *what);
+
+#define PLK_HIP(expr) ::plk::check_hip((expr), #expr)
+
+// ---------------------------------------------------------------------------
+// Device context: one stream, scratch for reductions, kernel statistics.
+// ---------------------------------------------------------------------------
+struct KernelStats {
+    double launches = 0;
+    double ms = 0;
+    double bytes = 0;
+};
+
+struct Device {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    double *red = nullptr;        // per-block FP64 partials
+    size_t red_cap = 0;           // doubles
+    double *red_out = nullptr;    // final reduction results
+    size_t red_out_cap = 0;
+    int num_cus = 256;
+};
+
+Device &device(int id);
+void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles);
+void set_device(int id);
+
+// Profiling: when enabled, launch() brackets kernels with HIP events.
+bool profiling_on();
+void record_kernel(const char *family, float ms, double bytes);
+
+struct ProfScope {
+    const char *family;
+    double bytes;
+    hipStream_t stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ProfScope(const char *f, double b, hipStream_t s);
+    ~ProfScope();
+};
+
+// ---------------------------------------------------------------------------
+// State vector.
+// ---------------------------------------------------------------------------
+struct State {
+    int n = 0;
+    int dtype = PLK_C128;
+    size_t len = 0;
+    void *d = nullptr;
+    int device = 0;
+    size_t elem_bytes() const { return dtype == PLK_C64 ? 8 : 16; }
+    size_t bytes() const { return len * elem_bytes(); }
+};
+
+// RAII temporary device state of the same shape.
+struct TmpState {
+    State s;
+    TmpState(int n, int dtype, int dev);
+    ~TmpState();
+    TmpState(const TmpState &) = delete;
+    TmpState &operator=(const TmpState &) = delete;
+};
+
+// ---------------------------------------------------------------------------
+// Compiled operations (host side, FP64).
+// ---------------------------------------------------------------------------
+enum class OpKind { Identity, Dense, Parity };
+
+struct CompiledOp {
+    std::string name;
+    std::vector<size_t> wires;  // wires[0] = most significant local bit
+    OpKind kind = OpKind::Dense;
+    std::vector<cd> mat;        // Dense: 2^m x 2^m row-major
+    cd even{1, 0}, odd{1, 0};   // Parity: phase by parity of the wire bits
+};
+
+struct CompiledGen {
+    std::vector<size_t> wires;
+    OpKind kind = OpKind::Dense; // Dense or Parity (signs +1 / -1)
+    std::vector<cd> mat;
+    double scale = 0;
+};
+
+bool is_named_gate(const std::string &name);
+size_t named_gate_wires(const std::string &name); // 0 = any (MultiRZ)
+size_t named_gate_params(const std::string &name);
+bool has_generator(const std::string &name);
+
+// U (or U^dagger when inverse) of a named gate or of a matrix op.
+CompiledOp compile_op(const std::string &name, const std::vector<size_t> &wires,
+                      bool inverse, const std::vector<double> &params,
+                      const std::vector<cd> &matrix, int num_qubits);
+CompiledGen compile_generator(const std::string &name,
+                              const std::vector<size_t> &wires, int num_qubits);
+
+// ---------------------------------------------------------------------------
+// Kernel launchers (implemented in the .hip files).
+// ---------------------------------------------------------------------------
+constexpr int SEG_KMAX = 3;      // wires per register-fused segment
+constexpr int SEG_PMAX = 4;      // trainable ops per segment
+constexpr int SEG_NJMAX = 8;     // observable states per launch
+constexpr int SEG_BLOCK = 256;
+
+// One fused operation as read by the segment kernel. `cls` selects the
+// compile-time local-position case; U and G are row-major complex matrices
+// in the state's precision (4x4 for two-wire ops, 2x2 in the first 8 reals
+// for one-wire ops).
+template <typename T> struct SegOp {
+    int32_t cls;
+    int32_t slot; // trainable slot in [0, SEG_PMAX) or -1
+    int32_t pad[2];
+    T U[32];
+    T G[32];
+};
+
+struct SegLaunch {
+    int n;              // qubits
+    int K;              // segment wires
+    int bits[SEG_KMAX]; // ascending bit positions of the segment wires
+    int nops;
+    int nslots;
+    int write_phi;
+    int nj;             // observable states in this launch
+};
+
+// Forward/backward fused segment over phi and nj observable states. Writes
+// per-block FP64 partials [slot][j][block] into dev.red when nj > 0 and
+// nslots > 0; returns the grid size used.
+int launch_segment(Device &dev, const State &phi, void *const *h_ptrs,
+                   const void *d_ops, const SegLaunch &L);
+// Largest nj accepted by launch_segment for a dtype and segment width.
+int seg_max_nj(int dtype, int K);
+// Reduces the per-block partials of launch_segment into
+// jac[(j0 + j) * ld + col[s]] = coef[s] * sum_blocks (device jac buffer).
+void launch_segment_reduce(Device &dev, int grid, int nslots, int nj, int j0,
+                           const int *cols, const double *coefs, double *d_jac,
+                           int ld);
+
+void launch_dense(Device &dev, const State &s, const CompiledOp &op);
+void launch_dense_matrix(Device &dev, const State &s,
+                         const std::vector<size_t> &wires,
+                         const std::vector<cd> &mat);
+void launch_parity(Device &dev, const State &s, const std::vector<size_t> &wires,
+                   cd even, cd odd);
+
+void launch_fill_zero(Device &dev, const State &s);
+void launch_set_basis(Device &dev, const State &s, size_t index);
+void launch_scatter(Device &dev, const State &s, const size_t *d_idx,
+                    const void *d_vals, size_t count);
+void launch_copy(Device &dev, const State &dst, const State &src);
+void launch_axpy(Device &dev, const State &y, double alpha, const State &x);
+
+// sum_i conj(x_j[i]) y[i] for j < nx, FP64; results (re, im) pairs in host out.
+void inner_products(Device &dev, const State *const *xs, int nx, const State &y,
+                    double *out_re_im);
+
+// Pauli-word sums: terms grouped by X mask.
+struct PauliTerm {
+    uint64_t x = 0;  // bits flipped (X or Y)
+    uint64_t zy = 0; // bits with a sign (Z or Y)
+    cd coeff{0, 0};  // includes (-i)^{#Y}
+};
+void launch_pauli_apply(Device &dev, const State &out, const State &in,
+                        const std::vector<PauliTerm> &terms);
+double pauli_expval(Device &dev, const State &s,
+                    const std::vector<PauliTerm> &terms);
+
+// CSR matrix (device copies made per call).
+double csr_expval(Device &dev, const State &s, const double *data,
+                  const int64_t *indices, const int64_t *indptr, size_t nnz,
+                  size_t rows);
+void csr_spmv(Device &dev, const State &out, const State &in, const double *data,
+              const int64_t *indices, const int64_t *indptr, size_t nnz,
+              size_t rows);
+
+} // namespace plk
+
+// Opaque handle types of the C ABI.
+struct plk_state {
+    plk::State s;
+};
+
+namespace plk {
+void profiling_enable_impl(bool on);
+KernelStats profiling_get_impl(const std::string &kernel);
+std::string profiling_names_impl();
+} // namespace plk

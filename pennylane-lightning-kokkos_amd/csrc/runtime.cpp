@@ -1,0 +1,185 @@
+// Device context, error plumbing and kernel statistics.
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <unordered_map>
+
+#include "plk_internal.hpp"
+
+namespace plk {
+
+[[noreturn]] void raise(int code, const std::string &msg) { throw Error{code, msg}; }
+
+void check_hip(hipError_t e, const char *what) {
+    if (e != hipSuccess) {
+        const int code = (e == hipErrorOutOfMemory) ? PLK_ERR_OOM : PLK_ERR_HIP;
+        raise(code, std::string("HIP error ") + hipGetErrorString(e) + " in " + what);
+    }
+}
+
+namespace {
+std::mutex g_dev_mutex;
+std::map<int, std::unique_ptr<Device>> g_devices;
+} // namespace
+
+void set_device(int id) { PLK_HIP(hipSetDevice(id)); }
+
+Device &device(int id) {
+    std::lock_guard<std::mutex> lock(g_dev_mutex);
+    auto it = g_devices.find(id);
+    if (it != g_devices.end()) {
+        set_device(id);
+        return *it->second;
+    }
+    int count = 0;
+    PLK_HIP(hipGetDeviceCount(&count));
+    if (id < 0 || id >= count) {
+        raise(PLK_ERR_INVALID, "device id " + std::to_string(id) + " out of range (" +
+                                   std::to_string(count) + " devices)");
+    }
+    auto d = std::make_unique<Device>();
+    d->id = id;
+    set_device(id);
+    PLK_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+    int cus = 0;
+    PLK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, id));
+    d->num_cus = cus > 0 ? cus : 256;
+    Device &ref = *d;
+    g_devices[id] = std::move(d);
+    ensure_red(ref, static_cast<size_t>(ref.num_cus) * 8 * SEG_PMAX * SEG_NJMAX, 256);
+    return ref;
+}
+
+void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles) {
+    if (partial_doubles > d.red_cap) {
+        PLK_HIP(hipStreamSynchronize(d.stream));
+        if (d.red) {
+            PLK_HIP(hipFree(d.red));
+            d.red = nullptr;
+        }
+        PLK_HIP(hipMalloc(&d.red, partial_doubles * sizeof(double)));
+        d.red_cap = partial_doubles;
+    }
+    if (out_doubles > d.red_out_cap) {
+        PLK_HIP(hipStreamSynchronize(d.stream));
+        if (d.red_out) {
+            PLK_HIP(hipFree(d.red_out));
+            d.red_out = nullptr;
+        }
+        PLK_HIP(hipMalloc(&d.red_out, out_doubles * sizeof(double)));
+        d.red_out_cap = out_doubles;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel statistics. Events are recorded around launches without blocking;
+// their durations are resolved when statistics are read.
+// ---------------------------------------------------------------------------
+namespace {
+struct Pending {
+    std::string family;
+    double bytes;
+    hipEvent_t e0, e1;
+};
+std::mutex g_prof_mutex;
+bool g_prof_on = false;
+std::vector<Pending> g_pending;
+std::map<std::string, KernelStats> g_stats;
+
+void resolve_pending() {
+    for (auto &p : g_pending) {
+        PLK_HIP(hipEventSynchronize(p.e1));
+        float ms = 0.f;
+        PLK_HIP(hipEventElapsedTime(&ms, p.e0, p.e1));
+        auto &s = g_stats[p.family];
+        s.launches += 1;
+        s.ms += ms;
+        s.bytes += p.bytes;
+        PLK_HIP(hipEventDestroy(p.e0));
+        PLK_HIP(hipEventDestroy(p.e1));
+    }
+    g_pending.clear();
+}
+} // namespace
+
+bool profiling_on() { return g_prof_on; }
+
+ProfScope::ProfScope(const char *f, double b, hipStream_t s) : family(f), bytes(b), stream(s) {
+    if (!g_prof_on) {
+        return;
+    }
+    PLK_HIP(hipEventCreate(&e0));
+    PLK_HIP(hipEventCreate(&e1));
+    PLK_HIP(hipEventRecord(e0, stream));
+}
+
+ProfScope::~ProfScope() {
+    if (!e0) {
+        return;
+    }
+    if (hipEventRecord(e1, stream) != hipSuccess) {
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        return;
+    }
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    g_pending.push_back({family, bytes, e0, e1});
+}
+
+void record_kernel(const char *family, float ms, double bytes) {
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    auto &s = g_stats[family];
+    s.launches += 1;
+    s.ms += ms;
+    s.bytes += bytes;
+}
+
+TmpState::TmpState(int n, int dtype, int dev) {
+    s.n = n;
+    s.dtype = dtype;
+    s.len = size_t{1} << n;
+    s.device = dev;
+    PLK_HIP(hipMalloc(&s.d, s.bytes()));
+}
+
+TmpState::~TmpState() {
+    if (s.d) {
+        // Kernels enqueued on this buffer must finish before it is released.
+        (void)hipDeviceSynchronize();
+        (void)hipFree(s.d);
+    }
+}
+
+} // namespace plk
+
+namespace plk {
+
+void profiling_enable_impl(bool on) {
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    resolve_pending();
+    g_stats.clear();
+    g_prof_on = on;
+}
+
+KernelStats profiling_get_impl(const std::string &kernel) {
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    resolve_pending();
+    auto it = g_stats.find(kernel);
+    return it == g_stats.end() ? KernelStats{} : it->second;
+}
+
+std::string profiling_names_impl() {
+    std::lock_guard<std::mutex> lock(g_prof_mutex);
+    resolve_pending();
+    std::string s;
+    for (auto &kv : g_stats) {
+        if (!s.empty()) {
+            s += ",";
+        }
+        s += kv.first;
+    }
+    return s;
+}
+
+} // namespace plk

@@ -1,0 +1,197 @@
+"""Circuit, state and observable builders shared by the CPU and GPU tests.
+
+Observables are described by plain tuples so the same description can be
+turned into an oracle object (checker) and into a C-ABI handle (product):
+  ("named", name, wires) | ("herm", matrix, wires) | ("tensor", [obs...])
+  | ("ham", coeffs, [obs...]) | ("sparse", data, indices, indptr, wires)
+"""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle as O
+
+# name -> (wires, params); wires 0 means "any number" (MultiRZ)
+GATES = {
+    "Identity": (1, 0), "PauliX": (1, 0), "PauliY": (1, 0), "PauliZ": (1, 0),
+    "Hadamard": (1, 0), "S": (1, 0), "T": (1, 0), "PhaseShift": (1, 1),
+    "RX": (1, 1), "RY": (1, 1), "RZ": (1, 1), "Rot": (1, 3),
+    "CNOT": (2, 0), "SWAP": (2, 0), "CY": (2, 0), "CZ": (2, 0),
+    "ControlledPhaseShift": (2, 1), "CRX": (2, 1), "CRY": (2, 1), "CRZ": (2, 1),
+    "CRot": (2, 3), "IsingXX": (2, 1), "IsingXY": (2, 1), "IsingYY": (2, 1),
+    "IsingZZ": (2, 1), "SingleExcitation": (2, 1), "SingleExcitationMinus": (2, 1),
+    "SingleExcitationPlus": (2, 1), "CSWAP": (3, 0), "Toffoli": (3, 0),
+    "DoubleExcitation": (4, 1), "DoubleExcitationMinus": (4, 1),
+    "DoubleExcitationPlus": (4, 1), "MultiRZ": (0, 1),
+}
+
+# Parametric gates with a generator (one parameter each).
+GEN_GATES = [g for g, (w, p) in GATES.items() if p == 1]
+
+
+def random_state(n: int, rng: np.random.Generator) -> np.ndarray:
+    v = rng.normal(size=2**n) + 1j * rng.normal(size=2**n)
+    return (v / np.linalg.norm(v)).astype(np.complex128)
+
+
+def random_unitary(dim: int, rng: np.random.Generator) -> np.ndarray:
+    z = (rng.normal(size=(dim, dim)) + 1j * rng.normal(size=(dim, dim))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diag(r)
+    return q * (d / np.abs(d))
+
+
+def random_hermitian(dim: int, rng: np.random.Generator) -> np.ndarray:
+    a = rng.normal(size=(dim, dim)) + 1j * rng.normal(size=(dim, dim))
+    return (a + a.conj().T) / 2
+
+
+def gate_wires(name: str, n: int, rng: np.random.Generator, k_multi: int = 3):
+    w, _ = GATES[name]
+    if w == 0:
+        w = min(n, k_multi)
+    return [int(x) for x in rng.choice(n, size=w, replace=False)]
+
+
+def random_ops(n: int, count: int, rng: np.random.Generator, names=None,
+               allow_inverse=True, with_matrix=False):
+    """Random op list (single-parameter gates only, as the adjoint requires)."""
+    names = names or [g for g in GATES if GATES[g][1] <= 1 and GATES[g][0] <= n]
+    ops_n, ops_p, ops_w, ops_i, ops_m = [], [], [], [], []
+    for _ in range(count):
+        g = names[rng.integers(len(names))]
+        w = gate_wires(g, n, rng)
+        p = [float(rng.uniform(0, 2 * np.pi))] if GATES[g][1] == 1 else []
+        ops_n.append(g)
+        ops_p.append(p)
+        ops_w.append(w)
+        ops_i.append(bool(allow_inverse and rng.integers(2)))
+        ops_m.append([])
+    if with_matrix:
+        k = int(rng.integers(1, min(3, n) + 1))
+        w = [int(x) for x in rng.choice(n, size=k, replace=False)]
+        pos = int(rng.integers(len(ops_n) + 1))
+        ops_n.insert(pos, "QubitUnitary")
+        ops_p.insert(pos, [])
+        ops_w.insert(pos, w)
+        ops_i.insert(pos, bool(allow_inverse and rng.integers(2)))
+        ops_m.insert(pos, random_unitary(2**k, rng).reshape(-1))
+    return ops_n, ops_p, ops_w, ops_i, ops_m
+
+
+def random_pauli_word(n: int, rng, max_weight=4):
+    k = int(rng.integers(1, min(max_weight, n) + 1))
+    wires = sorted(int(x) for x in rng.choice(n, size=k, replace=False))
+    names = [("PauliX", "PauliY", "PauliZ")[rng.integers(3)] for _ in wires]
+    if k == 1:
+        return ("named", names[0], [wires[0]])
+    return ("tensor", [("named", nm, [w]) for nm, w in zip(names, wires)])
+
+
+def random_hamiltonian(n: int, terms: int, rng, max_weight=4):
+    coeffs = [float(c) for c in rng.normal(size=terms)]
+    return ("ham", coeffs, [random_pauli_word(n, rng, max_weight) for _ in range(terms)])
+
+
+def to_oracle(ob):
+    kind = ob[0]
+    if kind == "named":
+        return O.Named(ob[1], list(ob[2]))
+    if kind == "herm":
+        return O.Hermitian(np.asarray(ob[1]), list(ob[2]))
+    if kind == "tensor":
+        return O.Tensor([to_oracle(f) for f in ob[1]])
+    if kind == "ham":
+        return O.Hamiltonian(list(ob[1]), [to_oracle(t) for t in ob[2]])
+    if kind == "sparse":
+        return O.Sparse(ob[1], ob[2], ob[3], list(ob[4]))
+    raise ValueError(kind)
+
+
+def to_capi(capi, ob):
+    kind = ob[0]
+    if kind == "named":
+        return capi.Obs.named(ob[1], ob[2])
+    if kind == "herm":
+        return capi.Obs.hermitian(ob[1], ob[2])
+    if kind == "tensor":
+        return capi.Obs.tensor([to_capi(capi, f) for f in ob[1]])
+    if kind == "ham":
+        return capi.Obs.hamiltonian(ob[1], [to_capi(capi, t) for t in ob[2]])
+    if kind == "sparse":
+        return capi.Obs.sparse(ob[1], ob[2], ob[3], ob[4])
+    raise ValueError(kind)
+
+
+def dense_pauli_sparse(n: int, ham, rng=None):
+    """CSR (data, indices, indptr) of a Hamiltonian of Pauli words (numpy)."""
+    import scipy.sparse as sp
+
+    def word_matrix(ob):
+        mats = {"PauliX": np.array([[0, 1], [1, 0]], complex),
+                "PauliY": np.array([[0, -1j], [1j, 0]], complex),
+                "PauliZ": np.array([[1, 0], [0, -1]], complex),
+                "Identity": np.eye(2, dtype=complex)}
+        factors = {}
+        if ob[0] == "named":
+            factors[ob[2][0]] = mats[ob[1]]
+        else:
+            for f in ob[1]:
+                factors[f[2][0]] = mats[f[1]]
+        M = sp.identity(1, dtype=complex, format="csr")
+        for w in range(n):
+            M = sp.kron(M, sp.csr_matrix(factors.get(w, np.eye(2, dtype=complex))), format="csr")
+        return M
+
+    H = None
+    for c, t in zip(ham[1], ham[2]):
+        term = c * word_matrix(t)
+        H = term if H is None else H + term
+    H = sp.csr_matrix(H)
+    H.sort_indices()
+    return H.data.astype(np.complex128), H.indices.astype(np.int64), H.indptr.astype(np.int64)
+
+
+# ----------------------------------------------------------------------------
+# Benchmark workloads (seeded).
+# ----------------------------------------------------------------------------
+def c1_workload(seed=20261202):
+    """12 qubits; 4 layers of RX, RY, RZ on every wire, then CNOT ring; Z on each wire."""
+    rng = np.random.default_rng(seed)
+    n = 12
+    names, params, wires = [], [], []
+    for _ in range(4):
+        for g in ("RX", "RY", "RZ"):
+            for w in range(n):
+                names.append(g)
+                params.append([float(rng.uniform(0, 2 * np.pi))])
+                wires.append([w])
+        for w in range(n):
+            names.append("CNOT")
+            params.append([])
+            wires.append([w, (w + 1) % n])
+    obs = [("named", "PauliZ", [w]) for w in range(n)]
+    return n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs
+
+
+def c3_workload(n=24, layers=10, seed=20261204, ham_terms=32):
+    """Strongly entangling layers: Rot(phi, theta, omega) on every wire as
+    RZ, RY, RZ, then CNOT(i, (i + r) mod n) with r = (layer mod (n-1)) + 1.
+    Observables: PauliZ on wires 0..7 plus one random Pauli Hamiltonian."""
+    rng = np.random.default_rng(seed)
+    names, params, wires = [], [], []
+    for layer in range(layers):
+        for w in range(n):
+            phi, theta, omega = (float(x) for x in rng.uniform(0, 2 * np.pi, size=3))
+            for g, a in (("RZ", phi), ("RY", theta), ("RZ", omega)):
+                names.append(g)
+                params.append([a])
+                wires.append([w])
+        r = (layer % (n - 1)) + 1
+        for w in range(n):
+            names.append("CNOT")
+            params.append([])
+            wires.append([w, (w + r) % n])
+    obs = [("named", "PauliZ", [w]) for w in range(min(8, n))]
+    obs.append(random_hamiltonian(n, ham_terms, rng))
+    return n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs
