@@ -99,6 +99,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--c64-steps", type=int, default=3)
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="only copy one 24-qubit complex128 state to another 10 times "
+                         "(256 MiB read + 256 MiB written per copy): the calibration "
+                         "reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path "
@@ -109,6 +113,13 @@ def main():
     from pennylane_lightning_kokkos_amd import _capi as capi
 
     capi.load()
+    if args.probe_copy:
+        a, b = capi.State(24), capi.State(24)
+        for _ in range(10):
+            b.copy_from(a)
+        b.synchronize()
+        print(json.dumps({"probe": "copy", "bytes_read_per_copy": 2**28, "bytes_written_per_copy": 2**28}))
+        return
     n, ops, obs = c3_workload(n=args.qubits)
     log(f"bench: {n} qubits, {len(ops[0])} ops, {len(obs)} observables; complex128 warmup")
     jac, dt, stats, ntp = run_gpu(capi, n, ops, obs, np.complex128, args.steps, args.warmup)

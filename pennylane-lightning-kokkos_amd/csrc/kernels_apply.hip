@@ -13,213 +13,9 @@
 // 199-236).
 #include <cstdio>
 
-#include "plk_internal.hpp"
+#include "device_common.hpp"
 
 namespace plk {
-
-template <typename T> struct Cx {
-    T r, i;
-};
-
-template <typename T>
-__device__ __forceinline__ Cx<T> cmul(const Cx<T> &a, const Cx<T> &b) {
-    return {a.r * b.r - a.i * b.i, a.r * b.i + a.i * b.r};
-}
-
-// c + a * b
-template <typename T>
-__device__ __forceinline__ Cx<T> cmad(const Cx<T> &a, const Cx<T> &b,
-                                      const Cx<T> &c) {
-    return {fma(a.r, b.r, fma(-a.i, b.i, c.r)), fma(a.r, b.i, fma(a.i, b.r, c.i))};
-}
-
-// Im(conj(h) * g) in FP64.
-template <typename T>
-__device__ __forceinline__ double im_conj_mul(const Cx<T> &h, const Cx<T> &g) {
-    return static_cast<double>(h.r) * static_cast<double>(g.i) -
-           static_cast<double>(h.i) * static_cast<double>(g.r);
-}
-
-template <typename T> struct VecOf;
-template <> struct VecOf<float> {
-    using type = float2;
-};
-template <> struct VecOf<double> {
-    using type = double2;
-};
-
-template <typename T>
-__device__ __forceinline__ Cx<T> ld(const Cx<T> *p, uint64_t i) {
-    using V = typename VecOf<T>::type;
-    const V v = reinterpret_cast<const V *>(p)[i];
-    return {v.x, v.y};
-}
-
-template <typename T>
-__device__ __forceinline__ void st(Cx<T> *p, uint64_t i, const Cx<T> &c) {
-    using V = typename VecOf<T>::type;
-    V v;
-    v.x = c.r;
-    v.y = c.i;
-    reinterpret_cast<V *>(p)[i] = v;
-}
-
-// Element access through a byte offset from a uniform base. With 32-bit
-// offsets the compiler emits SGPR-base + VGPR-offset addressing, so one VGPR
-// per amplitude position serves every state of the group.
-template <typename T, typename Off>
-__device__ __forceinline__ Cx<T> ldo(const Cx<T> *p, Off byte_off) {
-    using V = typename VecOf<T>::type;
-    const V v = *reinterpret_cast<const V *>(reinterpret_cast<const char *>(p) + byte_off);
-    return {v.x, v.y};
-}
-
-template <typename T, typename Off>
-__device__ __forceinline__ void sto(Cx<T> *p, Off byte_off, const Cx<T> &c) {
-    using V = typename VecOf<T>::type;
-    V v;
-    v.x = c.r;
-    v.y = c.i;
-    *reinterpret_cast<V *>(reinterpret_cast<char *>(p) + byte_off) = v;
-}
-
-__device__ __forceinline__ uint64_t insert_zero(uint64_t k, int b) {
-    const uint64_t low = k & ((uint64_t{1} << b) - 1);
-    return ((k >> b) << (b + 1)) | low;
-}
-
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        v += __shfl_xor(v, off, 64);
-    }
-    return v;
-}
-
-// ---------------------------------------------------------------------------
-// Register-level gate application on a group of 2^K amplitudes.
-// One-wire op on local bit Q with 2x2 matrix u.
-// ---------------------------------------------------------------------------
-template <typename T, int K, int Q>
-__device__ __forceinline__ void ap1(Cx<T> (&v)[1 << K], const Cx<T> (&u)[4]) {
-#pragma unroll
-    for (int i = 0; i < (1 << K); i++) {
-        if (i & (1 << Q)) {
-            continue;
-        }
-        const int j = i | (1 << Q);
-        const Cx<T> a = v[i], b = v[j];
-        v[i] = cmad(u[1], b, cmul(u[0], a));
-        v[j] = cmad(u[3], b, cmul(u[2], a));
-    }
-}
-
-// Two-wire op on local bits QA > QB; matrix index r = 2*bit(QA) + bit(QB).
-template <typename T, int K, int QA, int QB>
-__device__ __forceinline__ void ap2(Cx<T> (&v)[1 << K], const Cx<T> (&u)[16]) {
-#pragma unroll
-    for (int i = 0; i < (1 << K); i++) {
-        if (i & ((1 << QA) | (1 << QB))) {
-            continue;
-        }
-        const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
-        Cx<T> a[4];
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            a[r] = v[idx[r]];
-        }
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            Cx<T> s = cmul(u[4 * r], a[0]);
-#pragma unroll
-            for (int c = 1; c < 4; c++) {
-                s = cmad(u[4 * r + c], a[c], s);
-            }
-            v[idx[r]] = s;
-        }
-    }
-}
-
-// gf = G f for a one-wire generator on local bit Q (computed once per op).
-template <typename T, int K, int Q>
-__device__ __forceinline__ void gen1(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1 << K],
-                                     const Cx<T> (&g)[4]) {
-#pragma unroll
-    for (int i = 0; i < (1 << K); i++) {
-        if (i & (1 << Q)) {
-            continue;
-        }
-        const int j = i | (1 << Q);
-        gf[i] = cmad(g[1], f[j], cmul(g[0], f[i]));
-        gf[j] = cmad(g[3], f[j], cmul(g[2], f[i]));
-    }
-}
-
-template <typename T, int K, int QA, int QB>
-__device__ __forceinline__ void gen2(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1 << K],
-                                     const Cx<T> (&g)[16]) {
-#pragma unroll
-    for (int i = 0; i < (1 << K); i++) {
-        if (i & ((1 << QA) | (1 << QB))) {
-            continue;
-        }
-        const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            Cx<T> t = cmul(g[4 * r], f[idx[0]]);
-#pragma unroll
-            for (int c = 1; c < 4; c++) {
-                t = cmad(g[4 * r + c], f[idx[c]], t);
-            }
-            gf[idx[r]] = t;
-        }
-    }
-}
-
-// sum over the group of Im(conj(h) * gf), accumulated in FP64.
-template <typename T, int D>
-__device__ __forceinline__ double im_dot(const Cx<T> (&h)[D], const Cx<T> (&gf)[D]) {
-    double s = 0.0;
-#pragma unroll
-    for (int i = 0; i < D; i++) {
-        s += im_conj_mul(h[i], gf[i]);
-    }
-    return s;
-}
-
-template <int V> using IC = std::integral_constant<int, V>;
-
-// Maps the host-computed class of an op to (wire count, QA, QB).
-// K=1: 0 -> (1,0). K=2: 0,1 -> one wire on bit 0/1; 2 -> (1,0).
-// K=3: 0,1,2 -> one wire; 3 -> (1,0); 4 -> (2,0); 5 -> (2,1).
-template <int K, typename F>
-__device__ __forceinline__ void dispatch_cls(int cls, F &&f) {
-    if constexpr (K == 1) {
-        f(IC<1>{}, IC<0>{}, IC<0>{});
-    } else if constexpr (K == 2) {
-        switch (cls) {
-        case 0: f(IC<1>{}, IC<0>{}, IC<0>{}); break;
-        case 1: f(IC<1>{}, IC<1>{}, IC<0>{}); break;
-        default: f(IC<2>{}, IC<1>{}, IC<0>{}); break;
-        }
-    } else {
-        switch (cls) {
-        case 0: f(IC<1>{}, IC<0>{}, IC<0>{}); break;
-        case 1: f(IC<1>{}, IC<1>{}, IC<0>{}); break;
-        case 2: f(IC<1>{}, IC<2>{}, IC<0>{}); break;
-        case 3: f(IC<2>{}, IC<1>{}, IC<0>{}); break;
-        case 4: f(IC<2>{}, IC<2>{}, IC<0>{}); break;
-        default: f(IC<2>{}, IC<2>{}, IC<1>{}); break;
-        }
-    }
-}
-
-template <typename T>
-using ConstSegOp = __attribute__((address_space(4))) const SegOp<T>;
-
-template <typename T> struct SegPtrs {
-    Cx<T> *p[SEG_NJMAX];
-};
 
 template <typename T, int K, int NJ, typename Off>
 __global__ void __launch_bounds__(SEG_BLOCK)
