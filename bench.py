@@ -31,6 +31,22 @@ for _p in (os.path.join(REPO, "pennylane-lightning-kokkos_amd"), os.path.join(RE
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+TRAFFIC_JSON = os.path.join(REPO, "profiles", "adjoint_segment_traffic.json")
+
+
+def measured_traffic():
+    """HBM bytes per adjoint-segment launch measured by rocprofv3 --pmc
+    (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), valid
+    only while the kernel sources are the ones that were profiled."""
+    import glob
+    import hashlib
+
+    if not os.path.exists(TRAFFIC_JSON):
+        return None
+    rec = json.load(open(TRAFFIC_JSON))
+    src = sorted(glob.glob(os.path.join(REPO, "pennylane-lightning-kokkos_amd", "csrc", "*")))
+    h = hashlib.sha256(b"".join(open(p, "rb").read() for p in src)).hexdigest()[:16]
+    return rec["traffic_bytes_per_launch"] if rec.get("source_hash") == h else None
 
 
 def log(msg):
@@ -134,7 +150,7 @@ def main():
         avg_bytes = seg["bytes"] / seg["launches"]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
                 "kernel": "seg_kernel (fused adjoint segment)",
                 "avg_launch_ms": round(avg_ms, 4), "avg_launch_bytes": avg_bytes,
                 "launches_per_step": seg["launches"] / args.steps}
