@@ -3,6 +3,7 @@
 // point and the reference member it replaces.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <set>
@@ -179,9 +180,28 @@ struct Segment {
     int nslots = 0;
 };
 
+// A tile pass: bundles of ops on at most two tile bits each, all inside the
+// pass's T tile bits.
+struct Bundle {
+    int K = 1;
+    int qa = 0, qb = 0; // tile-local positions, qa > qb when K == 2
+    std::vector<PlanOp> ops;
+    int nslots = 0;
+};
+
+struct Pass {
+    std::vector<int> tb; // ascending global bit positions
+    std::vector<Bundle> bundles;
+    int nslots = 0;
+    int cols[TILE_PMAX] = {0};
+    double coefs[TILE_PMAX] = {0};
+};
+
 struct Step {
-    bool fused = false;
+    bool fused = false; // register segment
+    bool tiled = false; // LDS tile pass
     Segment seg;
+    Pass pass;
     PlanOp single;
 };
 
@@ -233,11 +253,200 @@ std::vector<Step> plan_steps(std::vector<PlanOp> &&ops) {
     return steps;
 }
 
+std::vector<int> op_bits(const PlanOp &p, int n) {
+    std::vector<int> b;
+    for (auto w : p.U.wires) {
+        b.push_back(n - 1 - static_cast<int>(w));
+    }
+    return b;
+}
+
+void prepare(PlanOp &p) {
+    densify(p.U);
+    if (p.trainable) {
+        densify(p.G);
+    }
+}
+
+constexpr size_t kPassMaxOps = 512;
+constexpr size_t kLookahead = 512;
+
+// Splits a pass's ops (in order) into bundles of <= 2 tile bits.
+void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
+    std::map<int, int> pos;
+    for (size_t q = 0; q < ps.tb.size(); q++) {
+        pos[ps.tb[q]] = static_cast<int>(q);
+    }
+    Bundle cur;
+    std::set<int> cb; // tile positions of the current bundle
+    auto flush = [&]() {
+        if (!cur.ops.empty()) {
+            std::vector<int> v(cb.begin(), cb.end());
+            if (v.size() == 1) {
+                cur.K = 1;
+                cur.qa = v[0];
+            } else {
+                cur.K = 2;
+                cur.qa = v[1];
+                cur.qb = v[0];
+            }
+            ps.bundles.push_back(std::move(cur));
+        }
+        cur = Bundle{};
+        cb.clear();
+    };
+    for (auto &p : ops) {
+        std::set<int> nb = cb;
+        for (int b : op_bits(p, n)) {
+            nb.insert(pos.at(b));
+        }
+        if (nb.size() > 2 || cur.ops.size() >= kSegMaxOps ||
+            (p.trainable && cur.nslots >= SEG_PMAX)) {
+            flush();
+            nb.clear();
+            for (int b : op_bits(p, n)) {
+                nb.insert(pos.at(b));
+            }
+        }
+        cb = nb;
+        if (p.trainable) {
+            cur.nslots++;
+            ps.cols[ps.nslots] = p.col;
+            ps.coefs[ps.nslots] = p.coef;
+            ps.nslots++;
+        }
+        cur.ops.push_back(std::move(p));
+    }
+    flush();
+}
+
+// Groups operations into LDS tile passes of T bits that always contain the
+// lowest L bits. Operations on disjoint wires commute, so an operation may be
+// taken ahead of skipped ones it shares no wire with; an unfusable operation
+// is a barrier and runs on its own.
+std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L) {
+    std::vector<PlanOp> rem;
+    for (auto &p : ops_in) {
+        if (p.U.kind == OpKind::Identity && !p.trainable) {
+            continue;
+        }
+        prepare(p);
+        rem.push_back(std::move(p));
+    }
+    std::vector<Step> steps;
+    size_t head = 0;
+    while (head < rem.size()) {
+        if (!fusable(rem[head])) {
+            Step s;
+            s.single = std::move(rem[head]);
+            steps.push_back(std::move(s));
+            head++;
+            continue;
+        }
+        std::set<int> bits;
+        for (int b = 0; b < L; b++) {
+            bits.insert(b);
+        }
+        std::set<size_t> blocked;
+        std::vector<PlanOp> taken;
+        std::vector<PlanOp> rest;
+        int nslots = 0;
+        size_t k = head;
+        for (; k < rem.size(); k++) {
+            PlanOp &p = rem[k];
+            if (!fusable(p) || k - head >= kLookahead) {
+                break;
+            }
+            bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < TILE_PMAX);
+            for (auto w : p.U.wires) {
+                ok = ok && !blocked.count(w);
+            }
+            std::set<int> nb = bits;
+            for (int b : op_bits(p, n)) {
+                nb.insert(b);
+            }
+            ok = ok && nb.size() <= static_cast<size_t>(T);
+            if (ok) {
+                bits = nb;
+                nslots += p.trainable ? 1 : 0;
+                taken.push_back(std::move(p));
+            } else {
+                blocked.insert(p.U.wires.begin(), p.U.wires.end());
+                rest.push_back(std::move(p));
+            }
+        }
+        for (; k < rem.size(); k++) {
+            rest.push_back(std::move(rem[k]));
+        }
+        // Pad the tile with the lowest unused bits.
+        for (int b = 0; static_cast<int>(bits.size()) < T && b < n; b++) {
+            bits.insert(b);
+        }
+        Step s;
+        s.tiled = true;
+        s.pass.tb.assign(bits.begin(), bits.end());
+        make_bundles(s.pass, std::move(taken), n);
+        steps.push_back(std::move(s));
+        rem = std::move(rest);
+        head = 0;
+    }
+    return steps;
+}
+
+int low_bits(int dtype) { return dtype == PLK_C64 ? 4 : 3; }
+
+int env_int(const char *name, int dflt) {
+    const char *v = std::getenv(name);
+    return v && *v ? std::atoi(v) : dflt;
+}
+
+// Tile width of backward (adjoint) passes; 0 selects register segments.
+int tile_bits_adjoint(int n, int dtype) {
+    if (env_int("PLK_TILES", 1) == 0) {
+        return 0;
+    }
+    const int T = std::min(env_int("PLK_TILE_BITS_ADJ", 10), n - 8);
+    return T >= low_bits(dtype) + 2 ? T : 0;
+}
+
+int tile_bits_forward(int n, int dtype) {
+    if (env_int("PLK_TILES", 1) == 0) {
+        return 0;
+    }
+    const int T = std::min(env_int("PLK_TILE_BITS_FWD", dtype == PLK_C64 ? 13 : 12), n - 8);
+    return T >= low_bits(dtype) + 2 ? T : 0;
+}
+
+// Largest observable chunk whose tiles fit the LDS budget.
+int tile_max_nj(int dtype, int T) {
+    int best = 1;
+    for (int nj = 1; nj <= 8; nj++) {
+        if (tile_nj_supported(dtype, nj) &&
+            tile_lds_bytes(dtype, T, nj, TILE_PMAX) <= TILE_LDS_MAX &&
+            nj <= (dtype == PLK_C64 ? 6 : 4)) {
+            best = nj;
+        }
+    }
+    return best;
+}
+
+// n_obs split into balanced chunks of at most njmax.
+std::vector<int> chunk_sizes(size_t n_obs, int njmax) {
+    const size_t k = (n_obs + njmax - 1) / njmax;
+    std::vector<int> out;
+    for (size_t c = 0; c < k; c++) {
+        out.push_back(static_cast<int>(n_obs / k + (c < n_obs % k ? 1 : 0)));
+    }
+    return out;
+}
+
 struct SegMeta {
     SegLaunch L{};
     size_t first = 0; // index of the first SegOp in the uploaded table
     int cols[SEG_PMAX] = {0};
     double coefs[SEG_PMAX] = {0};
+    TileLaunch TL{};
+    size_t first_bundle = 0;
 };
 
 template <typename T>
@@ -308,14 +517,72 @@ SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
     return meta;
 }
 
-// Device table of segment descriptors for a whole plan (one upload).
+template <typename T>
+void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
+                std::vector<TileBundle> &btab, SegMeta &meta) {
+    TileLaunch &TL = meta.TL;
+    TL.n = n;
+    TL.T = static_cast<int>(ps.tb.size());
+    for (int q = 0; q < TL.T; q++) {
+        TL.tb[q] = ps.tb[q];
+    }
+    std::map<int, int> pos;
+    for (size_t q = 0; q < ps.tb.size(); q++) {
+        pos[ps.tb[q]] = static_cast<int>(q);
+    }
+    meta.first_bundle = btab.size();
+    int slot0 = 0;
+    for (const auto &b : ps.bundles) {
+        TileBundle tb{};
+        tb.K = b.K;
+        tb.qa = b.qa;
+        tb.qb = b.qb;
+        tb.first = static_cast<int32_t>(table.size());
+        tb.nops = static_cast<int32_t>(b.ops.size());
+        tb.nslots = b.nslots;
+        tb.slot0 = slot0;
+        int slot = 0;
+        for (const auto &p : b.ops) {
+            SegOp<T> d{};
+            d.slot = -1;
+            const auto &w = p.U.wires;
+            bool swapb = false;
+            if (w.size() == 1) {
+                const int q = pos.at(n - 1 - static_cast<int>(w[0]));
+                d.cls = (b.K == 1 || q == b.qb) ? 0 : 1;
+            } else {
+                const int q0 = pos.at(n - 1 - static_cast<int>(w[0]));
+                swapb = (q0 != b.qa);
+                d.cls = 2;
+            }
+            put_mat(d.U, p.U.mat, swapb);
+            if (p.trainable) {
+                put_mat(d.G, p.G.mat, swapb);
+                d.slot = slot++;
+            }
+            table.push_back(d);
+        }
+        slot0 += b.nslots;
+        btab.push_back(tb);
+    }
+    TL.nbundles = static_cast<int>(ps.bundles.size());
+    TL.nslots = ps.nslots;
+}
+
+// Device tables of segment/pass descriptors for a whole plan (one upload).
 struct SegTable {
     void *d = nullptr;
     size_t elem = 0;
+    void *bundles = nullptr;
     ~SegTable() {
-        if (d) {
+        if (d || bundles) {
             (void)hipDeviceSynchronize();
+        }
+        if (d) {
             (void)hipFree(d);
+        }
+        if (bundles) {
+            (void)hipFree(bundles);
         }
     }
 };
@@ -323,10 +590,15 @@ struct SegTable {
 template <typename T>
 std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable &tab) {
     std::vector<SegOp<T>> table;
+    std::vector<TileBundle> btab;
     std::vector<SegMeta> metas;
     for (const auto &s : steps) {
         if (s.fused) {
             metas.push_back(build_segment<T>(s.seg, n, table));
+        } else if (s.tiled) {
+            SegMeta m;
+            build_pass<T>(s.pass, n, table, btab, m);
+            metas.push_back(m);
         } else {
             metas.push_back(SegMeta{});
         }
@@ -335,6 +607,11 @@ std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable
     if (!table.empty()) {
         PLK_HIP(hipMalloc(&tab.d, table.size() * sizeof(SegOp<T>)));
         PLK_HIP(hipMemcpy(tab.d, table.data(), table.size() * sizeof(SegOp<T>),
+                          hipMemcpyHostToDevice));
+    }
+    if (!btab.empty()) {
+        PLK_HIP(hipMalloc(&tab.bundles, btab.size() * sizeof(TileBundle)));
+        PLK_HIP(hipMemcpy(tab.bundles, btab.data(), btab.size() * sizeof(TileBundle),
                           hipMemcpyHostToDevice));
     }
     return metas;
@@ -360,6 +637,13 @@ void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
             L.nslots = 0;
             const void *ops = static_cast<const char *>(tab.d) + metas[i].first * tab.elem;
             launch_segment(dev, s, nullptr, ops, L);
+        } else if (steps[i].tiled) {
+            TileLaunch TL = metas[i].TL;
+            TL.nj = 0;
+            TL.write_phi = 1;
+            TL.nslots = 0;
+            const void *b = static_cast<const TileBundle *>(tab.bundles) + metas[i].first_bundle;
+            launch_tile(dev, s, nullptr, b, tab.d, TL);
         } else {
             apply_single(dev, s, steps[i].single.U);
         }
@@ -487,13 +771,17 @@ struct OpsData {
 };
 
 // Forward plan of the whole op list (applyOperations, AdjointDiffKokkos.hpp:213-226).
-std::vector<Step> forward_plan(const OpsData &ops, int n) {
+std::vector<Step> forward_plan(const OpsData &ops, int n, int dtype) {
     std::vector<PlanOp> v;
     for (size_t i = 0; i < ops.names.size(); i++) {
         PlanOp p;
         p.U = compile_op(ops.names[i], ops.wires[i], ops.inverses[i], ops.params[i],
                          ops.matrices[i], n);
         v.push_back(std::move(p));
+    }
+    const int T = tile_bits_forward(n, dtype);
+    if (T > 0) {
+        return plan_tiles(std::move(v), n, T, low_bits(dtype));
     }
     return plan_steps(std::move(v));
 }
@@ -557,12 +845,15 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     if (n_obs == 0) {
         return;
     }
-    const std::vector<Step> steps = plan_steps(std::move(rev));
+    const int T_adj = tile_bits_adjoint(n, ref.dtype);
+    const std::vector<Step> steps =
+        T_adj > 0 ? plan_tiles(std::move(rev), n, T_adj, low_bits(ref.dtype))
+                  : plan_steps(std::move(rev));
 
     TmpState lambda(n, ref.dtype, ref.device);
     launch_copy(dev, lambda.s, ref);
     if (apply_ops) {
-        run_forward(dev, lambda.s, forward_plan(ops, n));
+        run_forward(dev, lambda.s, forward_plan(ops, n, ref.dtype));
     }
     std::vector<std::unique_ptr<TmpState>> H;
     for (size_t j = 0; j < n_obs; j++) {
@@ -608,6 +899,21 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
                 if (L.nslots > 0) {
                     launch_segment_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0), m.cols,
                                           m.coefs, d_jac, static_cast<int>(n_tp));
+                }
+                j0 += nj;
+            }
+        } else if (st.tiled) {
+            const SegMeta &m = metas[i];
+            const void *b = static_cast<const TileBundle *>(tab.bundles) + m.first_bundle;
+            size_t j0 = 0;
+            for (int nj : chunk_sizes(n_obs, tile_max_nj(ref.dtype, m.TL.T))) {
+                TileLaunch TL = m.TL;
+                TL.nj = nj;
+                TL.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+                const int grid = launch_tile(dev, lambda.s, hptr.data() + j0, b, tab.d, TL);
+                if (TL.nslots > 0) {
+                    launch_tile_reduce(dev, grid, TL.nslots, nj, static_cast<int>(j0),
+                                       st.pass.cols, st.pass.coefs, d_jac, static_cast<int>(n_tp));
                 }
                 j0 += nj;
             }
@@ -904,7 +1210,7 @@ int plk_apply_ops(plk_state *sv, const plk_ops *ops) {
             raise(PLK_ERR_INVALID, "null ops handle");
         }
         Device &dev = dev_of(s);
-        run_forward(dev, s, forward_plan(ops->data, s.n));
+        run_forward(dev, s, forward_plan(ops->data, s.n, s.dtype));
     })
 }
 

@@ -212,4 +212,98 @@ template <typename T> struct SegPtrs {
     Cx<T> *p[SEG_NJMAX];
 };
 
+// Applies nops fused operations (descriptors in device memory, read through
+// the constant address space so the uniform loads become scalar loads) to
+// the register group f of phi and the groups h[j] of NJ observable states.
+// For a trainable op the FP64 inner products Im<h_j | G f> are added to
+// acc[slot][j] before the op is applied.
+template <typename T, int K, int NJ>
+__device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? NJ : 1][1 << K],
+                                        double (&acc)[SEG_PMAX][NJ > 0 ? NJ : 1],
+                                        const SegOp<T> *ops, int nops) {
+    constexpr int D = 1 << K;
+    for (int o = 0; o < nops; o++) {
+        // Descriptors are read through the constant address space so that
+        // the uniform loads become scalar loads into SGPRs.
+        const ConstSegOp<T> &op = reinterpret_cast<const ConstSegOp<T> *>(
+            (const __attribute__((address_space(4))) char *)(ops))[o];
+        const int cls = op.cls;
+        const int slot = op.slot;
+        dispatch_cls<K>(cls, [&](auto M, auto QA, auto QB) {
+            constexpr int m = decltype(M)::value;
+            constexpr int qa = decltype(QA)::value;
+            constexpr int qb = decltype(QB)::value;
+            if constexpr (m == 1) {
+                if constexpr (NJ > 0) {
+                    if (slot >= 0) {
+                        Cx<T> gm[4];
+#pragma unroll
+                        for (int e = 0; e < 4; e++) {
+                            gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
+                        }
+                        Cx<T> gf[D];
+                        gen1<T, K, qa>(gf, f, gm);
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) {
+                            const double d = im_dot(h[j], gf);
+#pragma unroll
+                            for (int s = 0; s < SEG_PMAX; s++) {
+                                if (s == slot) {
+                                    acc[s][j] += d;
+                                }
+                            }
+                        }
+                    }
+                }
+                Cx<T> u[4];
+#pragma unroll
+                for (int e = 0; e < 4; e++) {
+                    u[e] = {op.U[2 * e], op.U[2 * e + 1]};
+                }
+                ap1<T, K, qa>(f, u);
+                if constexpr (NJ > 0) {
+#pragma unroll
+                    for (int j = 0; j < NJ; j++) {
+                        ap1<T, K, qa>(h[j], u);
+                    }
+                }
+            } else {
+                if constexpr (NJ > 0) {
+                    if (slot >= 0) {
+                        Cx<T> gm[16];
+#pragma unroll
+                        for (int e = 0; e < 16; e++) {
+                            gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
+                        }
+                        Cx<T> gf[D];
+                        gen2<T, K, qa, qb>(gf, f, gm);
+#pragma unroll
+                        for (int j = 0; j < NJ; j++) {
+                            const double d = im_dot(h[j], gf);
+#pragma unroll
+                            for (int s = 0; s < SEG_PMAX; s++) {
+                                if (s == slot) {
+                                    acc[s][j] += d;
+                                }
+                            }
+                        }
+                    }
+                }
+                Cx<T> u[16];
+#pragma unroll
+                for (int e = 0; e < 16; e++) {
+                    u[e] = {op.U[2 * e], op.U[2 * e + 1]};
+                }
+                ap2<T, K, qa, qb>(f, u);
+                if constexpr (NJ > 0) {
+#pragma unroll
+                    for (int j = 0; j < NJ; j++) {
+                        ap2<T, K, qa, qb>(h[j], u);
+                    }
+                }
+            }
+        });
+    }
+}
+
 } // namespace plk

@@ -74,88 +74,7 @@ __global__ void __launch_bounds__(SEG_BLOCK)
                 }
             }
         }
-        for (int o = 0; o < L.nops; o++) {
-            // Descriptors are read through the constant address space so that
-            // the uniform loads become scalar loads into SGPRs.
-            const ConstSegOp<T> &op = reinterpret_cast<const ConstSegOp<T> *>(
-                (const __attribute__((address_space(4))) char *)(ops))[o];
-            const int cls = op.cls;
-            const int slot = op.slot;
-            dispatch_cls<K>(cls, [&](auto M, auto QA, auto QB) {
-                constexpr int m = decltype(M)::value;
-                constexpr int qa = decltype(QA)::value;
-                constexpr int qb = decltype(QB)::value;
-                if constexpr (m == 1) {
-                    if constexpr (NJ > 0) {
-                        if (slot >= 0) {
-                            Cx<T> gm[4];
-#pragma unroll
-                            for (int e = 0; e < 4; e++) {
-                                gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
-                            }
-                            Cx<T> gf[D];
-                            gen1<T, K, qa>(gf, f, gm);
-#pragma unroll
-                            for (int j = 0; j < NJ; j++) {
-                                const double d = im_dot(h[j], gf);
-#pragma unroll
-                                for (int s = 0; s < SEG_PMAX; s++) {
-                                    if (s == slot) {
-                                        acc[s][j] += d;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    Cx<T> u[4];
-#pragma unroll
-                    for (int e = 0; e < 4; e++) {
-                        u[e] = {op.U[2 * e], op.U[2 * e + 1]};
-                    }
-                    ap1<T, K, qa>(f, u);
-                    if constexpr (NJ > 0) {
-#pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            ap1<T, K, qa>(h[j], u);
-                        }
-                    }
-                } else {
-                    if constexpr (NJ > 0) {
-                        if (slot >= 0) {
-                            Cx<T> gm[16];
-#pragma unroll
-                            for (int e = 0; e < 16; e++) {
-                                gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
-                            }
-                            Cx<T> gf[D];
-                            gen2<T, K, qa, qb>(gf, f, gm);
-#pragma unroll
-                            for (int j = 0; j < NJ; j++) {
-                                const double d = im_dot(h[j], gf);
-#pragma unroll
-                                for (int s = 0; s < SEG_PMAX; s++) {
-                                    if (s == slot) {
-                                        acc[s][j] += d;
-                                    }
-                                }
-                            }
-                        }
-                    }
-                    Cx<T> u[16];
-#pragma unroll
-                    for (int e = 0; e < 16; e++) {
-                        u[e] = {op.U[2 * e], op.U[2 * e + 1]};
-                    }
-                    ap2<T, K, qa, qb>(f, u);
-                    if constexpr (NJ > 0) {
-#pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            ap2<T, K, qa, qb>(h[j], u);
-                        }
-                    }
-                }
-            });
-        }
+        run_ops<T, K, NJ>(f, h, acc, ops, L.nops);
         if (L.write_phi) {
 #pragma unroll
             for (int i = 0; i < D; i++) {
@@ -373,7 +292,7 @@ int seg_max_nj(int dtype, int K) {
     if (dtype == PLK_C64) {
         return K == 1 ? 8 : 4;
     }
-    return K == 1 ? 8 : (K == 2 ? 4 : 2);
+    return K <= 2 ? 4 : 2;
 }
 
 namespace {
@@ -391,7 +310,7 @@ int seg_launch_k(Device &dev, const State &phi, void *const *h, const void *d_op
         }
         break;
     case 8:
-        if constexpr (K == 1) {
+        if constexpr (K == 1 && sizeof(T) == 4) {
             return seg_launch_t<T, K, 8>(dev, phi, h, d_ops, L);
         }
         break;

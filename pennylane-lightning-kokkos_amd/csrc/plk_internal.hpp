@@ -165,6 +165,36 @@ void launch_segment_reduce(Device &dev, int grid, int nslots, int nj, int j0,
                            const int *cols, const double *coefs, double *d_jac,
                            int ld);
 
+// LDS tile passes (kernels_tile.hip).
+constexpr int TILE_BLOCK = 256;
+constexpr int TILE_TMAX = 14;          // tile bits
+constexpr int TILE_PMAX = 24;          // trainable ops per pass
+constexpr int TILE_BLOCKS_PER_CU = 2;
+constexpr size_t TILE_LDS_MAX = 80 * 1024;
+
+// A bundle: ops[first .. first+nops) on tile-local bits qa (> qb when K == 2);
+// its trainable ops use pass slots slot0 .. slot0+nslots.
+struct TileBundle {
+    int32_t K, qa, qb, first, nops, nslots, slot0, pad;
+};
+
+struct TileLaunch {
+    int n;
+    int T;                // tile bits
+    int tb[TILE_TMAX];    // ascending global bit positions of the tile bits
+    int nbundles;
+    int nslots;           // trainable ops of the pass
+    int nj;
+    int write_phi;
+};
+
+size_t tile_lds_bytes(int dtype, int T, int nj, int nslots);
+bool tile_nj_supported(int dtype, int nj);
+int launch_tile(Device &dev, const State &phi, void *const *h_ptrs, const void *d_bundles,
+                const void *d_ops, const TileLaunch &L);
+void launch_tile_reduce(Device &dev, int grid, int nslots, int nj, int j0, const int *cols,
+                        const double *coefs, double *d_jac, int ld);
+
 void launch_dense(Device &dev, const State &s, const CompiledOp &op);
 void launch_dense_matrix(Device &dev, const State &s,
                          const std::vector<size_t> &wires,

@@ -243,3 +243,76 @@ def test_adjoint_errors(capi):
         capi.Obs.tensor([capi.Obs.named("PauliZ", [0]), capi.Obs.named("PauliX", [0])])
     with pytest.raises(capi.PLException, match="Generator does not exist"):
         s.apply_generator("CNOT", [0, 1])
+
+
+# ----------------------------------------------------------------------------
+# LDS tile passes (states of 14+ qubits use them; PLK_TILES=0 forces the
+# register-segment path for cross-checks).
+# ----------------------------------------------------------------------------
+def _layered_ops(n, layers, rng):
+    names, params, wires = [], [], []
+    for layer in range(layers):
+        for w in range(n):
+            for g in ("RZ", "RY", "RZ"):
+                names.append(g)
+                params.append([float(rng.uniform(0, 2 * np.pi))])
+                wires.append([w])
+        r = (layer % (n - 1)) + 1
+        for w in range(n):
+            names.append("CNOT")
+            params.append([])
+            wires.append([w, (w + r) % n])
+    return names, params, wires, [False] * len(names), [[] for _ in names]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [14, 16, 18])
+def test_adjoint_tiles_layered(capi, n, dtype):
+    rng = np.random.default_rng(300 + n)
+    ops = _layered_ops(n, 3, rng)
+    obs = [("named", "PauliZ", [0]), ("named", "PauliX", [n - 1]), random_hamiltonian(n, 6, rng),
+           ("tensor", [("named", "PauliY", [2]), ("named", "PauliZ", [n - 2])]), ("named", "PauliZ", [n // 2])]
+    npar = sum(1 for p in ops[1] if p)
+    ref, got = _adjoint_pair(capi, O.zero_state(n), ops, obs, list(range(npar)), True, dtype)
+    _assert_jac(ref, got, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("seed", range(3))
+def test_adjoint_tiles_random(capi, seed, dtype):
+    rng = np.random.default_rng(400 + seed)
+    n = [15, 17, 18][seed]
+    ops = random_ops(n, 120, rng, with_matrix=(seed != 1))
+    npar = sum(1 for p in ops[1] if p)
+    tp = sorted(int(x) for x in rng.choice(npar, size=npar - 3 * seed, replace=False))
+    obs = [("named", "PauliZ", [j % n]) for j in range(2 + 3 * seed)] + [random_hamiltonian(n, 5, rng)]
+    psi = random_state(n, rng)
+    ref, got = _adjoint_pair(capi, psi, ops, obs, tp, seed != 2, dtype)
+    _assert_jac(ref, got, dtype)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [20, 21])
+def test_apply_ops_tiles(capi, n, dtype):
+    rng = np.random.default_rng(500 + n)
+    names, params, wires, inv, mats = random_ops(n, 150, rng, with_matrix=True)
+    psi = random_state(n, rng)
+    ref = O.apply_ops(psi.copy(), O.Ops(names, params, wires, inv, mats))
+    s = _gpu_state(capi, psi, dtype)
+    s.apply_ops(capi.Ops(names, params, wires, inv, mats))
+    np.testing.assert_allclose(s.download(), ref, atol=50 * _tol(dtype))
+
+
+def test_tiles_match_segments(capi, monkeypatch):
+    rng = np.random.default_rng(600)
+    n = 18
+    ops = _layered_ops(n, 2, rng)
+    obs = [("named", "PauliZ", [3]), random_hamiltonian(n, 4, rng)]
+    npar = sum(1 for p in ops[1] if p)
+    s = _gpu_state(capi, O.zero_state(n), np.complex128)
+    cobs = [to_capi(capi, o) for o in obs]
+    cops = capi.Ops(*ops)
+    tiled = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
+    monkeypatch.setenv("PLK_TILES", "0")
+    segs = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
+    np.testing.assert_allclose(tiled, segs, rtol=1e-11, atol=1e-13)
