@@ -34,7 +34,7 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_JSON = os.path.join(REPO, "profiles", "adjoint_segment_traffic.json")
 
 
-def measured_traffic():
+def measured_traffic(family):
     """HBM bytes per adjoint-segment launch measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), valid
     only while the kernel sources are the ones that were profiled."""
@@ -44,6 +44,8 @@ def measured_traffic():
     if not os.path.exists(TRAFFIC_JSON):
         return None
     rec = json.load(open(TRAFFIC_JSON))
+    if rec.get("family") != family:
+        return None
     src = sorted(glob.glob(os.path.join(REPO, "pennylane-lightning-kokkos_amd", "csrc", "*")))
     h = hashlib.sha256(b"".join(open(p, "rb").read() for p in src)).hexdigest()[:16]
     return rec["traffic_bytes_per_launch"] if rec.get("source_hash") == h else None
@@ -143,15 +145,18 @@ def main():
     value = ntp * args.steps / dt
     log(f"bench: complex128 {ms:.1f} ms/step, {value:.1f} params/s")
 
-    seg = stats.get("adjoint_segment", {"launches": 0, "total_ms": 0, "bytes": 0})
+    fams = [k for k in stats if k.startswith("adjoint_")]
+    dom = max(fams, key=lambda k: stats[k]["total_ms"]) if fams else None
+    seg = stats.get(dom, {"launches": 0, "total_ms": 0, "bytes": 0})
     roof = None
     if seg["launches"]:
         avg_ms = seg["total_ms"] / seg["launches"]
         avg_bytes = seg["bytes"] / seg["launches"]
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(),
-                "kernel": "seg_kernel (fused adjoint segment)",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
+                "kernel": {"adjoint_tile": "tile_kernel (LDS tile pass, adjoint)",
+                           "adjoint_segment": "seg_kernel (register segment, adjoint)"}.get(dom, dom),
                 "avg_launch_ms": round(avg_ms, 4), "avg_launch_bytes": avg_bytes,
                 "launches_per_step": seg["launches"] / args.steps}
     breakdown = {k: {"ms_per_step": round(v["total_ms"] / args.steps, 3),

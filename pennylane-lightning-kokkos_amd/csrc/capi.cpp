@@ -462,6 +462,79 @@ void put_mat(T *dst, const std::vector<cd> &M, bool swap_bits) {
     }
 }
 
+// Picks the structure-specialized kernel path for a descriptor whose U (and
+// G) already hold the final, bit-permuted matrices of the op.
+template <typename T> void classify(SegOp<T> &d, size_t m) {
+    const int dim = m == 1 ? 2 : 4;
+    auto re = [&](const T *M, int r, int c) { return M[2 * (r * dim + c)]; };
+    auto im = [&](const T *M, int r, int c) { return M[2 * (r * dim + c) + 1]; };
+    bool diag = true, real = true;
+    for (int r = 0; r < dim; r++) {
+        for (int c = 0; c < dim; c++) {
+            if (r != c && (re(d.U, r, c) != 0 || im(d.U, r, c) != 0)) {
+                diag = false;
+            }
+            if (im(d.U, r, c) != 0) {
+                real = false;
+            }
+        }
+    }
+    // A transposition of two op-local indices with unit entries.
+    int moved[4], nm = 0;
+    bool perm = real;
+    for (int r = 0; r < dim && perm; r++) {
+        int ones = 0, col = -1;
+        for (int c = 0; c < dim; c++) {
+            const T v = re(d.U, r, c);
+            if (v == 1) {
+                ones++;
+                col = c;
+            } else if (v != 0) {
+                perm = false;
+            }
+        }
+        perm = perm && ones == 1;
+        if (perm && col != r) {
+            if (nm < 4) {
+                moved[nm] = r;
+            }
+            nm++;
+        }
+    }
+    if (diag) {
+        d.ukind = U_DIAG;
+    } else if (perm && nm == 2 && re(d.U, moved[0], moved[1]) == 1) {
+        d.ukind = U_SWAP;
+        d.swap = 4 * moved[0] + moved[1];
+    } else if (real) {
+        d.ukind = U_REAL;
+    } else {
+        d.ukind = U_DENSE;
+    }
+    d.gkind = G_DENSE;
+    if (d.slot >= 0) {
+        bool gdiag = true;
+        for (int r = 0; r < dim; r++) {
+            for (int c = 0; c < dim; c++) {
+                if (im(d.G, r, c) != 0 || (r != c && re(d.G, r, c) != 0)) {
+                    gdiag = false;
+                }
+            }
+        }
+        if (gdiag) {
+            d.gkind = G_DIAG;
+        } else if (m == 1 && re(d.G, 0, 0) == 0 && im(d.G, 0, 0) == 0 && re(d.G, 1, 1) == 0 &&
+                   im(d.G, 1, 1) == 0) {
+            if (re(d.G, 0, 1) == 1 && im(d.G, 0, 1) == 0 && re(d.G, 1, 0) == 1 && im(d.G, 1, 0) == 0) {
+                d.gkind = G_X;
+            } else if (re(d.G, 0, 1) == 0 && im(d.G, 0, 1) == -1 && re(d.G, 1, 0) == 0 &&
+                       im(d.G, 1, 0) == 1) {
+                d.gkind = G_Y;
+            }
+        }
+    }
+}
+
 template <typename T>
 SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
     SegMeta meta;
@@ -510,6 +583,7 @@ SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
             meta.coefs[slot] = p.coef;
             slot++;
         }
+        classify(d, w.size());
         table.push_back(d);
     }
     L.nops = static_cast<int>(seg.ops.size());
@@ -560,6 +634,7 @@ void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
                 put_mat(d.G, p.G.mat, swapb);
                 d.slot = slot++;
             }
+            classify(d, w.size());
             table.push_back(d);
         }
         slot0 += b.nslots;

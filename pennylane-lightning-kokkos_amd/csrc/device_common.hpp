@@ -212,6 +212,162 @@ template <typename T> struct SegPtrs {
     Cx<T> *p[SEG_NJMAX];
 };
 
+// Local index (within the op) of element i of a group: one-wire ops use bit
+// Q, two-wire ops r = 2 * bit(QA) + bit(QB).
+template <int M, int QA, int QB> __device__ __forceinline__ constexpr int op_index(int i) {
+    return M == 1 ? ((i >> QA) & 1) : ((((i >> QA) & 1) << 1) | ((i >> QB) & 1));
+}
+
+// Exchanges the elements whose op-local indices are A and B (A < B).
+template <typename T, int K, int M, int QA, int QB, int A, int B>
+__device__ __forceinline__ void swap_pair(Cx<T> (&v)[1 << K]) {
+#pragma unroll
+    for (int i = 0; i < (1 << K); i++) {
+        if (op_index<M, QA, QB>(i) != A) {
+            continue;
+        }
+        int j = i;
+        if (M == 1) {
+            j = i ^ (1 << QA);
+        } else {
+            if ((A ^ B) & 2) {
+                j ^= 1 << QA;
+            }
+            if ((A ^ B) & 1) {
+                j ^= 1 << QB;
+            }
+        }
+        const Cx<T> t = v[i];
+        v[i] = v[j];
+        v[j] = t;
+    }
+}
+
+// U applied to one group according to its structure.
+template <typename T, int K, int M, int QA, int QB>
+__device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int swap,
+                                           const __attribute__((address_space(4))) T *U) {
+    constexpr int D = 1 << K;
+    constexpr int dim = 1 << M;
+    if (ukind == U_DIAG) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const int r = op_index<M, QA, QB>(i);
+            v[i] = cmul(Cx<T>{U[2 * (r * dim + r)], U[2 * (r * dim + r) + 1]}, v[i]);
+        }
+    } else if (ukind == U_SWAP) {
+        if constexpr (M == 1) {
+            swap_pair<T, K, M, QA, QB, 0, 1>(v);
+        } else {
+            switch (swap) {
+            case 1: swap_pair<T, K, M, QA, QB, 0, 1>(v); break;
+            case 2: swap_pair<T, K, M, QA, QB, 0, 2>(v); break;
+            case 3: swap_pair<T, K, M, QA, QB, 0, 3>(v); break;
+            case 6: swap_pair<T, K, M, QA, QB, 1, 2>(v); break;
+            case 7: swap_pair<T, K, M, QA, QB, 1, 3>(v); break;
+            default: swap_pair<T, K, M, QA, QB, 2, 3>(v); break;
+            }
+        }
+    } else if (ukind == U_REAL) {
+        if constexpr (M == 1) {
+            const T m00 = U[0], m01 = U[2], m10 = U[4], m11 = U[6];
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                if (i & (1 << QA)) {
+                    continue;
+                }
+                const int j = i | (1 << QA);
+                const Cx<T> a = v[i], b = v[j];
+                v[i] = {fma(m00, a.r, m01 * b.r), fma(m00, a.i, m01 * b.i)};
+                v[j] = {fma(m10, a.r, m11 * b.r), fma(m10, a.i, m11 * b.i)};
+            }
+        } else {
+#pragma unroll
+            for (int i = 0; i < D; i++) {
+                if (i & ((1 << QA) | (1 << QB))) {
+                    continue;
+                }
+                const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
+                Cx<T> a[4];
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    a[r] = v[idx[r]];
+                }
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    T sr = U[2 * (4 * r)] * a[0].r, si = U[2 * (4 * r)] * a[0].i;
+#pragma unroll
+                    for (int c = 1; c < 4; c++) {
+                        sr = fma(U[2 * (4 * r + c)], a[c].r, sr);
+                        si = fma(U[2 * (4 * r + c)], a[c].i, si);
+                    }
+                    v[idx[r]] = {sr, si};
+                }
+            }
+        }
+    } else {
+        if constexpr (M == 1) {
+            Cx<T> u[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                u[e] = {U[2 * e], U[2 * e + 1]};
+            }
+            ap1<T, K, QA>(v, u);
+        } else {
+            Cx<T> u[16];
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                u[e] = {U[2 * e], U[2 * e + 1]};
+            }
+            ap2<T, K, QA, QB>(v, u);
+        }
+    }
+}
+
+// gf = G f for the group according to the generator's structure.
+template <typename T, int K, int M, int QA, int QB>
+__device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1 << K], int gkind,
+                                         const __attribute__((address_space(4))) T *G) {
+    constexpr int D = 1 << K;
+    constexpr int dim = 1 << M;
+    if (gkind == G_DIAG) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const int r = op_index<M, QA, QB>(i);
+            const T g = G[2 * (r * dim + r)];
+            gf[i] = {g * f[i].r, g * f[i].i};
+        }
+    } else if (M == 1 && gkind == G_X) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            gf[i] = f[i ^ (1 << QA)];
+        }
+    } else if (M == 1 && gkind == G_Y) {
+        // Y = [[0, -i], [i, 0]]: (Y f)_0 = -i f_1, (Y f)_1 = i f_0.
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const Cx<T> p = f[i ^ (1 << QA)];
+            gf[i] = ((i >> QA) & 1) ? Cx<T>{-p.i, p.r} : Cx<T>{p.i, -p.r};
+        }
+    } else {
+        if constexpr (M == 1) {
+            Cx<T> g[4];
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                g[e] = {G[2 * e], G[2 * e + 1]};
+            }
+            gen1<T, K, QA>(gf, f, g);
+        } else {
+            Cx<T> g[16];
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                g[e] = {G[2 * e], G[2 * e + 1]};
+            }
+            gen2<T, K, QA, QB>(gf, f, g);
+        }
+    }
+}
+
 // Applies nops fused operations (descriptors in device memory, read through
 // the constant address space so the uniform loads become scalar loads) to
 // the register group f of phi and the groups h[j] of NJ observable states.
@@ -223,83 +379,38 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
                                         const SegOp<T> *ops, int nops) {
     constexpr int D = 1 << K;
     for (int o = 0; o < nops; o++) {
-        // Descriptors are read through the constant address space so that
-        // the uniform loads become scalar loads into SGPRs.
         const ConstSegOp<T> &op = reinterpret_cast<const ConstSegOp<T> *>(
             (const __attribute__((address_space(4))) char *)(ops))[o];
         const int cls = op.cls;
         const int slot = op.slot;
-        dispatch_cls<K>(cls, [&](auto M, auto QA, auto QB) {
-            constexpr int m = decltype(M)::value;
-            constexpr int qa = decltype(QA)::value;
-            constexpr int qb = decltype(QB)::value;
-            if constexpr (m == 1) {
-                if constexpr (NJ > 0) {
-                    if (slot >= 0) {
-                        Cx<T> gm[4];
+        const int ukind = op.ukind;
+        const int gkind = op.gkind;
+        const int swp = op.swap;
+        dispatch_cls<K>(cls, [&](auto Mc, auto QAc, auto QBc) {
+            constexpr int m = decltype(Mc)::value;
+            constexpr int qa = decltype(QAc)::value;
+            constexpr int qb = decltype(QBc)::value;
+            if constexpr (NJ > 0) {
+                if (slot >= 0) {
+                    Cx<T> gf[D];
+                    gen_kind<T, K, m, qa, qb>(gf, f, gkind, op.G);
 #pragma unroll
-                        for (int e = 0; e < 4; e++) {
-                            gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
-                        }
-                        Cx<T> gf[D];
-                        gen1<T, K, qa>(gf, f, gm);
+                    for (int j = 0; j < NJ; j++) {
+                        const double d = im_dot(h[j], gf);
 #pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            const double d = im_dot(h[j], gf);
-#pragma unroll
-                            for (int s = 0; s < SEG_PMAX; s++) {
-                                if (s == slot) {
-                                    acc[s][j] += d;
-                                }
+                        for (int s = 0; s < SEG_PMAX; s++) {
+                            if (s == slot) {
+                                acc[s][j] += d;
                             }
                         }
                     }
                 }
-                Cx<T> u[4];
+            }
+            apply_kind<T, K, m, qa, qb>(f, ukind, swp, op.U);
+            if constexpr (NJ > 0) {
 #pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    u[e] = {op.U[2 * e], op.U[2 * e + 1]};
-                }
-                ap1<T, K, qa>(f, u);
-                if constexpr (NJ > 0) {
-#pragma unroll
-                    for (int j = 0; j < NJ; j++) {
-                        ap1<T, K, qa>(h[j], u);
-                    }
-                }
-            } else {
-                if constexpr (NJ > 0) {
-                    if (slot >= 0) {
-                        Cx<T> gm[16];
-#pragma unroll
-                        for (int e = 0; e < 16; e++) {
-                            gm[e] = {op.G[2 * e], op.G[2 * e + 1]};
-                        }
-                        Cx<T> gf[D];
-                        gen2<T, K, qa, qb>(gf, f, gm);
-#pragma unroll
-                        for (int j = 0; j < NJ; j++) {
-                            const double d = im_dot(h[j], gf);
-#pragma unroll
-                            for (int s = 0; s < SEG_PMAX; s++) {
-                                if (s == slot) {
-                                    acc[s][j] += d;
-                                }
-                            }
-                        }
-                    }
-                }
-                Cx<T> u[16];
-#pragma unroll
-                for (int e = 0; e < 16; e++) {
-                    u[e] = {op.U[2 * e], op.U[2 * e + 1]};
-                }
-                ap2<T, K, qa, qb>(f, u);
-                if constexpr (NJ > 0) {
-#pragma unroll
-                    for (int j = 0; j < NJ; j++) {
-                        ap2<T, K, qa, qb>(h[j], u);
-                    }
+                for (int j = 0; j < NJ; j++) {
+                    apply_kind<T, K, m, qa, qb>(h[j], ukind, swp, op.U);
                 }
             }
         });

@@ -134,10 +134,19 @@ constexpr int SEG_BLOCK = 256;
 // compile-time local-position case; U and G are row-major complex matrices
 // in the state's precision (4x4 for two-wire ops, 2x2 in the first 8 reals
 // for one-wire ops).
+// Structure of an operation's matrix, chosen on the host so the kernels skip
+// the multiplications a dense complex product would waste.
+enum SegUKind : int32_t { U_DENSE = 0, U_DIAG = 1, U_REAL = 2, U_SWAP = 3 };
+// Structure of a generator: dense, real diagonal, or (one wire) X or Y.
+enum SegGKind : int32_t { G_DENSE = 0, G_DIAG = 1, G_X = 2, G_Y = 3 };
+
 template <typename T> struct SegOp {
     int32_t cls;
-    int32_t slot; // trainable slot in [0, SEG_PMAX) or -1
-    int32_t pad[2];
+    int32_t slot;  // trainable slot in [0, SEG_PMAX) or -1
+    int32_t ukind; // SegUKind
+    int32_t gkind; // SegGKind
+    int32_t swap;  // U_SWAP: 4 * a + b, local indices a < b exchanged
+    int32_t pad[3];
     T U[32];
     T G[32];
 };
