@@ -79,6 +79,29 @@ __device__ __forceinline__ uint64_t insert_zero(uint64_t k, int b) {
     return ((k >> b) << (b + 1)) | low;
 }
 
+// Value of lane (i - SHIFT) of the same 16-lane row, 0 when that lane is
+// outside the row (DPP row_shr, VALU only).
+template <int SHIFT> __device__ __forceinline__ double row_shr(double v) {
+    static_assert(SHIFT >= 1 && SHIFT <= 15, "row_shr shift");
+    const uint64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b & 0xffffffffu), 0x110 + SHIFT,
+                                               0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), 0x110 + SHIFT, 0xF,
+                                               0xF, true);
+    return __longlong_as_double(static_cast<long long>(
+        (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo)));
+}
+
+// Inclusive prefix sum within each 16-lane row: the row's last lane
+// (lane % 16 == 15) receives the row total.
+__device__ __forceinline__ double row_total16(double v) {
+    v += row_shr<1>(v);
+    v += row_shr<2>(v);
+    v += row_shr<4>(v);
+    v += row_shr<8>(v);
+    return v;
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

@@ -84,16 +84,18 @@ __device__ __forceinline__ void run_bundle(Cx<T> *__restrict__ tile, int TS,
     }
     if constexpr (NJ > 0) {
         if (bd.nslots > 0) {
-            const int lane = threadIdx.x & 63;
-            const int wave = threadIdx.x >> 6;
+            // One partial per 16-lane row; each row's last lane owns its slot,
+            // so the accumulation order is fixed.
+            const bool owner = (threadIdx.x & 15) == 15;
+            const int row = threadIdx.x >> 4;
 #pragma unroll
             for (int s = 0; s < SEG_PMAX; s++) {
                 if (s < bd.nslots) {
 #pragma unroll
                     for (int j = 0; j < NJ; j++) {
-                        const double v = wave_sum(acc[s][j]);
-                        if (lane == 0) {
-                            lacc[(wave * pass_slots + bd.slot0 + s) * NJ + j] += v;
+                        const double v = row_total16(acc[s][j]);
+                        if (owner) {
+                            lacc[(row * pass_slots + bd.slot0 + s) * NJ + j] += v;
                         }
                     }
                 }
@@ -111,7 +113,7 @@ __global__ void __launch_bounds__(TILE_BLOCK)
     const int TS = 1 << L.T;
     Cx<T> *tile = reinterpret_cast<Cx<T> *>(smem);
     double *lacc = reinterpret_cast<double *>(smem + static_cast<size_t>(NS) * TS * sizeof(Cx<T>));
-    const int nacc = (TILE_BLOCK / 64) * L.nslots * (NJ > 0 ? NJ : 1);
+    const int nacc = TILE_ROWS * L.nslots * (NJ > 0 ? NJ : 1);
     for (int i = threadIdx.x; i < nacc; i += TILE_BLOCK) {
         lacc[i] = 0.0;
     }
@@ -172,7 +174,7 @@ __global__ void __launch_bounds__(TILE_BLOCK)
         for (int i = threadIdx.x; i < per; i += TILE_BLOCK) {
             double v = 0.0;
 #pragma unroll
-            for (int w = 0; w < TILE_BLOCK / 64; w++) {
+            for (int w = 0; w < TILE_ROWS; w++) {
                 v += lacc[w * per + i];
             }
             partials[static_cast<uint64_t>(i) * gridDim.x + blockIdx.x] = v;
@@ -294,7 +296,7 @@ int tile_launch_dtype(Device &dev, const State &phi, void *const *h, const void 
 size_t tile_lds_bytes(int dtype, int T, int nj, int nslots) {
     const size_t elem = dtype == PLK_C64 ? 8 : 16;
     return static_cast<size_t>(nj + 1) * (size_t{1} << T) * elem +
-           static_cast<size_t>(TILE_BLOCK / 64) * nslots * (nj > 0 ? nj : 1) * sizeof(double);
+           static_cast<size_t>(TILE_ROWS) * nslots * (nj > 0 ? nj : 1) * sizeof(double);
 }
 
 bool tile_nj_supported(int dtype, int nj) {

@@ -179,6 +179,7 @@ constexpr int TILE_BLOCK = 256;
 constexpr int TILE_TMAX = 14;          // tile bits
 constexpr int TILE_PMAX = 24;          // trainable ops per pass
 constexpr int TILE_BLOCKS_PER_CU = 2;
+constexpr int TILE_ROWS = TILE_BLOCK / 16; // 16-lane rows holding FP64 partials
 constexpr size_t TILE_LDS_MAX = 80 * 1024;
 
 // A bundle: ops[first .. first+nops) on tile-local bits qa (> qb when K == 2);
