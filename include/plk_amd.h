@@ -200,6 +200,26 @@ int plk_adjoint_jacobian(const plk_state *sv, const plk_obs *const *obs,
                          const size_t *trainable, size_t num_trainable,
                          int apply_operations, double *jac);
 
+/* Batched form of plk_adjoint_jacobian: num_circuits independent circuits,
+ * each applied to |0...0> of num_qubits[c] (1..20) qubits, each with one
+ * observable obs[c] (a sum of products of Identity / PauliX / PauliY /
+ * PauliZ / Hadamard), all run by one persistent launch on device_id in the
+ * precision dtype. Replaces a loop over circuits of
+ * AdjointJacobianKokkos::adjointJacobian (AdjointDiffKokkos.hpp:404-478)
+ * with apply_operations = true, the per-tape calls of the device's
+ * adjoint_jacobian (lightning_kokkos.py:621-687). Ops are limited to gates
+ * (and generators) on at most two wires.
+ * expvals[c] receives <obs[c]>; the Jacobian rows are packed: circuit c's
+ * row of num_trainable[c] entries starts after the rows of circuits 0..c-1.
+ * trainable == NULL makes every parametric op of every circuit trainable
+ * (row length = plk_ops_num_par_ops). Synchronous; FP64 results. */
+int plk_adjoint_batched(int dtype, int device_id, size_t num_circuits,
+                        const int *num_qubits, const plk_ops *const *ops,
+                        const plk_obs *const *obs,
+                        const size_t *const *trainable,
+                        const size_t *num_trainable, double *expvals,
+                        double *jac);
+
 /* ---- measurement support ---------------------------------------------- */
 
 /* Enables (1) or disables (0) per-kernel timing with HIP events on the

@@ -32,7 +32,7 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "kernels_apply.hip",
-               "kernels_measure.hip", "kernels_tile.hip"]
+               "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip"]
 
 
 def _run(cmd, verbose):

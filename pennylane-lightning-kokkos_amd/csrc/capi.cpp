@@ -6,6 +6,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <map>
+#include <queue>
 #include <set>
 #include <sstream>
 
@@ -193,8 +194,8 @@ struct Pass {
     std::vector<int> tb; // ascending global bit positions
     std::vector<Bundle> bundles;
     int nslots = 0;
-    int cols[TILE_PMAX] = {0};
-    double coefs[TILE_PMAX] = {0};
+    std::vector<int> cols;     // Jacobian column of each trainable op, in order
+    std::vector<double> coefs; // and its coefficient
 };
 
 struct Step {
@@ -311,8 +312,8 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
         cb = nb;
         if (p.trainable) {
             cur.nslots++;
-            ps.cols[ps.nslots] = p.col;
-            ps.coefs[ps.nslots] = p.coef;
+            ps.cols.push_back(p.col);
+            ps.coefs.push_back(p.coef);
             ps.nslots++;
         }
         cur.ops.push_back(std::move(p));
@@ -324,7 +325,8 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
 // lowest L bits. Operations on disjoint wires commute, so an operation may be
 // taken ahead of skipped ones it shares no wire with; an unfusable operation
 // is a barrier and runs on its own.
-std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L) {
+std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
+                             int pmax = TILE_PMAX) {
     std::vector<PlanOp> rem;
     for (auto &p : ops_in) {
         if (p.U.kind == OpKind::Identity && !p.trainable) {
@@ -357,7 +359,7 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L) 
             if (!fusable(p) || k - head >= kLookahead) {
                 break;
             }
-            bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < TILE_PMAX);
+            bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < pmax);
             for (auto w : p.U.wires) {
                 ok = ok && !blocked.count(w);
             }
@@ -861,6 +863,48 @@ std::vector<Step> forward_plan(const OpsData &ops, int n, int dtype) {
     return plan_steps(std::move(v));
 }
 
+// The reverse walk of AdjointDiffKokkos.hpp:432-470: U^dagger of every op
+// after the last trainable one, in reverse order, each trainable op carrying
+// its generator, Jacobian column and coefficient -2 s (inverse ? -1 : 1).
+// StatePrep / BasisState are skipped; ops with several parameters abort.
+std::vector<PlanOp> reverse_ops(const OpsData &ops, const std::vector<size_t> &tp, int n) {
+    const size_t n_tp = tp.size();
+    std::vector<PlanOp> rev;
+    size_t trainable_number = n_tp - 1;
+    size_t current_param_idx = ops.num_par_ops() - 1;
+    auto tp_it = tp.rbegin();
+    for (long op_idx = static_cast<long>(ops.names.size()) - 1; op_idx >= 0; op_idx--) {
+        const size_t o = static_cast<size_t>(op_idx);
+        if (ops.params[o].size() > 1) {
+            raise(PLK_ERR_INVALID,
+                  "The operation is not supported using the adjoint "
+                  "differentiation method");
+        }
+        if (ops.names[o] == "StatePrep" || ops.names[o] == "BasisState") {
+            continue;
+        }
+        if (tp_it == tp.rend()) {
+            break;
+        }
+        PlanOp p;
+        p.U = compile_op(ops.names[o], ops.wires[o], !ops.inverses[o], ops.params[o],
+                         ops.matrices[o], n);
+        if (!ops.params[o].empty()) {
+            if (current_param_idx == *tp_it) {
+                p.trainable = true;
+                p.G = compile_generator(ops.names[o], ops.wires[o], n);
+                p.coef = -2.0 * p.G.scale * (ops.inverses[o] ? -1.0 : 1.0);
+                p.col = static_cast<int>(trainable_number);
+                trainable_number--;
+                ++tp_it;
+            }
+            current_param_idx--;
+        }
+        rev.push_back(std::move(p));
+    }
+    return rev;
+}
+
 // ---------------------------------------------------------------------------
 // Adjoint Jacobian (AdjointDiffKokkos.hpp:404-478).
 //
@@ -882,41 +926,7 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     const int n = ref.n;
 
     // Reverse plan first: it validates the op list before any device work.
-    std::vector<PlanOp> rev;
-    {
-        size_t trainable_number = n_tp - 1;
-        size_t current_param_idx = ops.num_par_ops() - 1;
-        auto tp_it = tp.rbegin();
-        for (long op_idx = static_cast<long>(ops.names.size()) - 1; op_idx >= 0; op_idx--) {
-            const size_t o = static_cast<size_t>(op_idx);
-            if (ops.params[o].size() > 1) {
-                raise(PLK_ERR_INVALID,
-                      "The operation is not supported using the adjoint "
-                      "differentiation method");
-            }
-            if (ops.names[o] == "StatePrep" || ops.names[o] == "BasisState") {
-                continue;
-            }
-            if (tp_it == tp.rend()) {
-                break;
-            }
-            PlanOp p;
-            p.U = compile_op(ops.names[o], ops.wires[o], !ops.inverses[o], ops.params[o],
-                             ops.matrices[o], n);
-            if (!ops.params[o].empty()) {
-                if (current_param_idx == *tp_it) {
-                    p.trainable = true;
-                    p.G = compile_generator(ops.names[o], ops.wires[o], n);
-                    p.coef = -2.0 * p.G.scale * (ops.inverses[o] ? -1.0 : 1.0);
-                    p.col = static_cast<int>(trainable_number);
-                    trainable_number--;
-                    ++tp_it;
-                }
-                current_param_idx--;
-            }
-            rev.push_back(std::move(p));
-        }
-    }
+    std::vector<PlanOp> rev = reverse_ops(ops, tp, n);
     if (n_obs == 0) {
         return;
     }
@@ -988,7 +998,7 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
                 const int grid = launch_tile(dev, lambda.s, hptr.data() + j0, b, tab.d, TL);
                 if (TL.nslots > 0) {
                     launch_tile_reduce(dev, grid, TL.nslots, nj, static_cast<int>(j0),
-                                       st.pass.cols, st.pass.coefs, d_jac, static_cast<int>(n_tp));
+                                       st.pass.cols.data(), st.pass.coefs.data(), d_jac, static_cast<int>(n_tp));
                 }
                 j0 += nj;
             }
@@ -1022,6 +1032,248 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     for (auto &e : host_entries) {
         jac[e.first] = e.second;
     }
+}
+
+// ---------------------------------------------------------------------------
+// Batched circuits: independent circuits, each from |0...0> with one
+// observable, planned on the host into LDS tile passes and run by one
+// persistent launch (kernels_batch.hip).
+// ---------------------------------------------------------------------------
+struct BatchInput {
+    int n = 0;
+    const OpsData *ops = nullptr;
+    const ObsNode *obs = nullptr;
+    std::vector<size_t> tp;
+};
+
+int batch_tile_bits(int dtype) {
+    const int T = env_int("PLK_BATCH_TILE_BITS", dtype == PLK_C64 ? 12 : 11);
+    return std::max(low_bits(dtype) + 2, std::min(T, BATCH_TMAX));
+}
+
+// Host-side batch plan; every table is uploaded as one blob.
+template <typename T> struct BatchPlan {
+    std::vector<BatchCircuit> circuits;
+    std::vector<BatchPass> passes;
+    std::vector<TileBundle> bundles;
+    std::vector<SegOp<T>> ops;
+    std::vector<BatchTerm> terms;
+    std::vector<int32_t> slot_cols;
+    std::vector<double> slot_coefs;
+    std::vector<double> cost;
+    int64_t jac_len = 0;
+};
+
+template <typename T>
+void plan_batch_passes(BatchPlan<T> &bp, std::vector<Step> &&steps, const BatchCircuit &c,
+                       int Tt, bool forward, int &first, int &count) {
+    first = static_cast<int>(bp.passes.size());
+    if (forward && steps.empty() && !c.resident) {
+        Step s;
+        s.tiled = true;
+        for (int b = 0; b < Tt; b++) {
+            s.pass.tb.push_back(b);
+        }
+        steps.push_back(std::move(s));
+    }
+    for (auto &st : steps) {
+        if (!st.tiled) {
+            raise(PLK_ERR_UNSUPPORTED,
+                  "batched circuits support dense gates and generators on at most two wires");
+        }
+        SegMeta meta;
+        BatchPass P{};
+        P.bundle_first = static_cast<int32_t>(bp.bundles.size());
+        build_pass<T>(st.pass, c.n, bp.ops, bp.bundles, meta);
+        P.T = meta.TL.T;
+        P.bundle_count = meta.TL.nbundles;
+        P.nslots = st.pass.nslots;
+        P.slot_first = static_cast<int32_t>(bp.slot_cols.size());
+        for (int q = 0; q < P.T; q++) {
+            P.tb[q] = st.pass.tb[q];
+            P.tmask |= 1u << st.pass.tb[q];
+        }
+        for (int k = 0; k < P.nslots; k++) {
+            bp.slot_cols.push_back(st.pass.cols[k]);
+            bp.slot_coefs.push_back(st.pass.coefs[k]);
+        }
+        size_t ops_in_pass = 0;
+        for (const auto &b : st.pass.bundles) {
+            ops_in_pass += b.ops.size();
+        }
+        const double amps = std::ldexp(1.0, c.n);
+        bp.cost.back() += amps * (static_cast<double>(P.bundle_count) + 0.25 * ops_in_pass) *
+                              (forward ? 1.0 : 2.0) +
+                          (c.resident ? 0.0 : 8.0 * amps);
+        bp.passes.push_back(P);
+    }
+    count = static_cast<int>(bp.passes.size()) - first;
+    if (!c.resident && count > 0) {
+        if (forward) {
+            bp.passes[first].flags |= BP_INIT;
+        } else {
+            bp.passes.back().flags |= BP_NO_STORE;
+        }
+    }
+}
+
+template <typename T>
+void plan_batch_circuit(BatchPlan<T> &bp, const BatchInput &in, int tile_bits, int L) {
+    const int n = in.n;
+    BatchCircuit c{};
+    c.n = n;
+    c.resident = n <= tile_bits ? 1 : 0;
+    const int Tt = c.resident ? n : tile_bits;
+    const int Lc = c.resident ? 0 : L;
+    c.jac_offset = bp.jac_len;
+    bp.cost.push_back(0.0);
+
+    std::vector<PauliTerm> pt;
+    if (!to_pauli(*in.obs, n, pt)) {
+        raise(PLK_ERR_UNSUPPORTED,
+              "batched circuits support observables that are sums of Pauli words");
+    }
+    c.term_first = static_cast<int32_t>(bp.terms.size());
+    c.term_count = static_cast<int32_t>(pt.size());
+    for (const auto &t : pt) {
+        bp.terms.push_back(BatchTerm{t.x, t.zy, t.coeff.real(), t.coeff.imag()});
+    }
+    bp.cost.back() += std::ldexp(1.0, n) * (1.0 + 0.25 * static_cast<double>(pt.size()));
+
+    const OpsData &ops = *in.ops;
+    std::vector<PlanOp> rev = reverse_ops(ops, in.tp, n);
+    std::vector<PlanOp> fwd;
+    for (size_t i = 0; i < ops.names.size(); i++) {
+        PlanOp p;
+        p.U = compile_op(ops.names[i], ops.wires[i], ops.inverses[i], ops.params[i],
+                         ops.matrices[i], n);
+        fwd.push_back(std::move(p));
+    }
+    plan_batch_passes(bp, plan_tiles(std::move(fwd), n, Tt, Lc, BATCH_PMAX), c, Tt, true,
+                      c.fwd_first, c.fwd_count);
+    plan_batch_passes(bp, plan_tiles(std::move(rev), n, Tt, Lc, BATCH_PMAX), c, Tt, false,
+                      c.bwd_first, c.bwd_count);
+    bp.jac_len += static_cast<int64_t>(in.tp.size());
+    bp.circuits.push_back(c);
+}
+
+struct Blob {
+    std::vector<unsigned char> bytes;
+    template <typename V> size_t add(const std::vector<V> &v) {
+        const size_t off = (bytes.size() + 255) & ~size_t{255};
+        bytes.resize(off + v.size() * sizeof(V));
+        if (!v.empty()) {
+            std::memcpy(bytes.data() + off, v.data(), v.size() * sizeof(V));
+        }
+        return off;
+    }
+};
+
+template <typename T>
+void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double *expvals,
+                       double *jac) {
+    const int dtype = sizeof(T) == 4 ? PLK_C64 : PLK_C128;
+    const int tile_bits = batch_tile_bits(dtype);
+    BatchPlan<T> bp;
+    for (size_t i = 0; i < in.size(); i++) {
+        try {
+            plan_batch_circuit<T>(bp, in[i], tile_bits, low_bits(dtype));
+        } catch (const Error &e) {
+            raise(e.code, "circuit " + std::to_string(i) + ": " + e.msg);
+        }
+    }
+    if (in.empty()) {
+        return;
+    }
+    set_device(device_id);
+    Device &dev = device(device_id);
+
+    // Longest-processing-time-first packing of circuits onto workgroups.
+    const size_t lds = batch_lds_bytes(dtype, tile_bits) + sizeof(double) * 1024;
+    const int per_cu = std::max(1, std::min(8, static_cast<int>((160 * 1024) / lds)));
+    const int grid = static_cast<int>(
+        std::min<size_t>(in.size(), static_cast<size_t>(dev.num_cus) * per_cu));
+    std::vector<int32_t> by_cost(in.size());
+    for (size_t i = 0; i < in.size(); i++) {
+        by_cost[i] = static_cast<int32_t>(i);
+    }
+    std::stable_sort(by_cost.begin(), by_cost.end(),
+                     [&](int32_t a, int32_t b) { return bp.cost[a] > bp.cost[b]; });
+    using Load = std::pair<double, int>;
+    std::priority_queue<Load, std::vector<Load>, std::greater<Load>> heap;
+    for (int b = 0; b < grid; b++) {
+        heap.push({0.0, b});
+    }
+    std::vector<std::vector<int32_t>> lists(grid);
+    for (int32_t ci : by_cost) {
+        Load l = heap.top();
+        heap.pop();
+        lists[l.second].push_back(ci);
+        l.first += bp.cost[ci];
+        heap.push(l);
+    }
+    std::vector<int32_t> order, block_first{0};
+    std::vector<int64_t> block_scratch;
+    int64_t scratch_amps = 0;
+    for (int b = 0; b < grid; b++) {
+        int64_t need = 0;
+        for (int32_t ci : lists[b]) {
+            order.push_back(ci);
+            if (!bp.circuits[ci].resident) {
+                need = std::max<int64_t>(need, int64_t{2} << bp.circuits[ci].n);
+            }
+        }
+        block_first.push_back(static_cast<int32_t>(order.size()));
+        block_scratch.push_back(scratch_amps);
+        scratch_amps += need;
+    }
+
+    Blob blob;
+    const size_t o_circ = blob.add(bp.circuits);
+    const size_t o_order = blob.add(order);
+    const size_t o_bfirst = blob.add(block_first);
+    const size_t o_bscr = blob.add(block_scratch);
+    const size_t o_pass = blob.add(bp.passes);
+    const size_t o_bund = blob.add(bp.bundles);
+    const size_t o_ops = blob.add(bp.ops);
+    const size_t o_terms = blob.add(bp.terms);
+    const size_t o_cols = blob.add(bp.slot_cols);
+    const size_t o_coefs = blob.add(bp.slot_coefs);
+    const size_t tables = (blob.bytes.size() + 255) & ~size_t{255};
+    const size_t amp = 2 * sizeof(T);
+    const size_t o_scratch = tables;
+    const size_t o_exp = o_scratch + ((static_cast<size_t>(scratch_amps) * amp + 255) & ~size_t{255});
+    const size_t o_jac = o_exp + ((in.size() * sizeof(double) + 255) & ~size_t{255});
+    const size_t total = o_jac + static_cast<size_t>(bp.jac_len) * sizeof(double) + 256;
+    unsigned char *w = static_cast<unsigned char *>(ensure_work(dev, total));
+    PLK_HIP(hipMemcpyAsync(w, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice,
+                           dev.stream));
+
+    BatchArgs A{};
+    A.circuits = reinterpret_cast<const BatchCircuit *>(w + o_circ);
+    A.order = reinterpret_cast<const int32_t *>(w + o_order);
+    A.block_first = reinterpret_cast<const int32_t *>(w + o_bfirst);
+    A.block_scratch = reinterpret_cast<const int64_t *>(w + o_bscr);
+    A.passes = reinterpret_cast<const BatchPass *>(w + o_pass);
+    A.bundles = reinterpret_cast<const TileBundle *>(w + o_bund);
+    A.ops = w + o_ops;
+    A.terms = reinterpret_cast<const BatchTerm *>(w + o_terms);
+    A.slot_cols = reinterpret_cast<const int32_t *>(w + o_cols);
+    A.slot_coefs = reinterpret_cast<const double *>(w + o_coefs);
+    A.scratch = w + o_scratch;
+    A.expvals = reinterpret_cast<double *>(w + o_exp);
+    A.jac = reinterpret_cast<double *>(w + o_jac);
+    A.jac_len = bp.jac_len;
+    A.tile_bits = tile_bits;
+    A.grid = grid;
+    launch_batch(dev, dtype, A);
+    PLK_HIP(hipMemcpyAsync(expvals, A.expvals, in.size() * sizeof(double), hipMemcpyDeviceToHost,
+                           dev.stream));
+    if (bp.jac_len > 0) {
+        PLK_HIP(hipMemcpyAsync(jac, A.jac, static_cast<size_t>(bp.jac_len) * sizeof(double),
+                               hipMemcpyDeviceToHost, dev.stream));
+    }
+    PLK_HIP(hipStreamSynchronize(dev.stream));
 }
 
 std::vector<size_t> to_vec(const size_t *w, size_t n) {
@@ -1501,6 +1753,58 @@ int plk_adjoint_jacobian(const plk_state *sv, const plk_obs *const *obs, size_t 
         }
         std::vector<size_t> tp(trainable, trainable + num_trainable);
         adjoint_jacobian(s, ob, ops->data, tp, apply_operations != 0, jac);
+    })
+}
+
+int plk_adjoint_batched(int dtype, int device_id, size_t num_circuits, const int *num_qubits,
+                        const plk_ops *const *ops, const plk_obs *const *obs,
+                        const size_t *const *trainable, const size_t *num_trainable,
+                        double *expvals, double *jac) {
+    PLK_TRY({
+        if (dtype != PLK_C64 && dtype != PLK_C128) {
+            raise(PLK_ERR_INVALID, "dtype must be PLK_C64 or PLK_C128");
+        }
+        if (num_circuits > 0 && (!num_qubits || !ops || !obs || !expvals)) {
+            raise(PLK_ERR_INVALID, "null argument");
+        }
+        if (trainable && !num_trainable) {
+            raise(PLK_ERR_INVALID, "trainable lists need their lengths");
+        }
+        std::vector<BatchInput> in(num_circuits);
+        for (size_t i = 0; i < num_circuits; i++) {
+            const std::string who = "circuit " + std::to_string(i) + ": ";
+            if (num_qubits[i] < 1 || num_qubits[i] > BATCH_NMAX) {
+                raise(PLK_ERR_UNSUPPORTED, who + "batched circuits have 1.." +
+                                               std::to_string(BATCH_NMAX) + " qubits");
+            }
+            if (!ops[i]) {
+                raise(PLK_ERR_INVALID, who + "null ops handle");
+            }
+            in[i].n = num_qubits[i];
+            in[i].ops = &ops[i]->data;
+            in[i].obs = &O(obs[i]);
+            if (trainable) {
+                if (num_trainable[i] > 0 && !trainable[i]) {
+                    raise(PLK_ERR_INVALID, who + "null trainable list");
+                }
+                in[i].tp.assign(trainable[i], trainable[i] + num_trainable[i]);
+            } else {
+                for (size_t k = 0; k < ops[i]->data.num_par_ops(); k++) {
+                    in[i].tp.push_back(k);
+                }
+            }
+            if (in[i].tp.empty()) {
+                raise(PLK_ERR_INVALID, who + "No trainable parameters provided.");
+            }
+        }
+        if (num_circuits > 0 && !jac) {
+            raise(PLK_ERR_INVALID, "null jac buffer");
+        }
+        if (dtype == PLK_C64) {
+            adjoint_batched_t<float>(device_id, in, expvals, jac);
+        } else {
+            adjoint_batched_t<double>(device_id, in, expvals, jac);
+        }
     })
 }
 

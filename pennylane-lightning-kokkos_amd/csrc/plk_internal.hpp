@@ -49,10 +49,14 @@ struct Device {
     double *red_out = nullptr;    // final reduction results
     size_t red_out_cap = 0;
     int num_cus = 256;
+    void *work = nullptr;         // grow-only device workspace
+    size_t work_cap = 0;          // bytes
 };
 
 Device &device(int id);
 void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles);
+// Grows d.work to at least `bytes` (synchronizes the stream when it grows).
+void *ensure_work(Device &d, size_t bytes);
 void set_device(int id);
 
 // Profiling: when enabled, launch() brackets kernels with HIP events.
@@ -204,6 +208,67 @@ int launch_tile(Device &dev, const State &phi, void *const *h_ptrs, const void *
                 const void *d_ops, const TileLaunch &L);
 void launch_tile_reduce(Device &dev, int grid, int nslots, int nj, int j0, const int *cols,
                         const double *coefs, double *d_jac, int ld);
+
+// Batched circuits (kernels_batch.hip). One workgroup runs one circuit at a
+// time from |0...0>: forward passes, h = H phi and <H>, reverse passes with
+// the FP64 Im<h|G phi> of every trainable op. A pass is a set of T tile bits
+// and bundles of ops on <= 2 tile positions. States of n <= tile_bits qubits
+// stay in LDS (resident); larger ones live in a per-workgroup slice of a
+// global buffer and are streamed through LDS one 2^T-amplitude tile at a time.
+constexpr int BATCH_BLOCK = 256;
+constexpr int BATCH_PMAX = 64;  // trainable ops per pass
+constexpr int BATCH_TMAX = 12;  // tile bits
+constexpr int BATCH_NMAX = 20;  // qubits per circuit
+
+enum BatchPassFlags { BP_INIT = 1, BP_NO_STORE = 2 };
+
+struct BatchPass {
+    int32_t T;                 // tile bits (== n for a resident circuit)
+    int32_t flags;             // BatchPassFlags
+    int32_t bundle_first, bundle_count;
+    int32_t nslots;            // trainable ops of the pass
+    int32_t slot_first;        // their Jacobian columns / coefficients
+    uint32_t tmask;            // mask of the tile bits
+    int32_t pad;
+    int32_t tb[BATCH_TMAX];    // ascending global bit of each tile position
+};
+
+struct BatchCircuit {
+    int32_t n;
+    int32_t resident;
+    int32_t fwd_first, fwd_count; // passes
+    int32_t bwd_first, bwd_count;
+    int32_t term_first, term_count;
+    int64_t jac_offset;           // first Jacobian entry of the circuit
+    int64_t pad;
+};
+
+struct BatchTerm {
+    uint64_t x, zy; // Pauli word: flipped bits, signed bits
+    double cr, ci;  // coefficient, including (-i)^{#Y}
+};
+
+struct BatchArgs {
+    const BatchCircuit *circuits;
+    const int32_t *order;        // circuit indices, grouped by workgroup
+    const int32_t *block_first;  // grid + 1 offsets into order
+    const int64_t *block_scratch; // amplitude offset of each workgroup's slice
+    const BatchPass *passes;
+    const TileBundle *bundles;
+    const void *ops;             // SegOp<T>
+    const BatchTerm *terms;
+    const int32_t *slot_cols;
+    const double *slot_coefs;
+    void *scratch;               // Cx<T> amplitudes
+    double *expvals;
+    double *jac;
+    int64_t jac_len;
+    int32_t tile_bits;
+    int32_t grid;
+};
+
+size_t batch_lds_bytes(int dtype, int tile_bits);
+void launch_batch(Device &dev, int dtype, const BatchArgs &A);
 
 void launch_dense(Device &dev, const State &s, const CompiledOp &op);
 void launch_dense_matrix(Device &dev, const State &s,

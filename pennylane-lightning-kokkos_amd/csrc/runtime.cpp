@@ -72,6 +72,23 @@ void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles) {
     }
 }
 
+void *ensure_work(Device &d, size_t bytes) {
+    if (bytes > d.work_cap) {
+        PLK_HIP(hipStreamSynchronize(d.stream));
+        if (d.work) {
+            PLK_HIP(hipFree(d.work));
+            d.work = nullptr;
+            d.work_cap = 0;
+        }
+        if (hipMalloc(&d.work, bytes) != hipSuccess) {
+            (void)hipGetLastError();
+            raise(PLK_ERR_OOM, "device workspace of " + std::to_string(bytes) + " bytes");
+        }
+        d.work_cap = bytes;
+    }
+    return d.work;
+}
+
 // ---------------------------------------------------------------------------
 // Kernel statistics. Events are recorded around launches without blocking;
 // their durations are resolved when statistics are read.

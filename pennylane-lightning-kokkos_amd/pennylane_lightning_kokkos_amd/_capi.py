@@ -72,6 +72,10 @@ SIGNATURES = {
     "plk_ops_num_par_ops": (ctypes.c_int, [_vp, _c_size_p]),
     "plk_adjoint_jacobian": (ctypes.c_int, [_vp, ctypes.POINTER(_vp), ctypes.c_size_t, _vp, _c_size_p,
                                             ctypes.c_size_t, ctypes.c_int, _c_double_p]),
+    "plk_adjoint_batched": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                           ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_vp),
+                                           ctypes.POINTER(_vp), ctypes.POINTER(_c_size_p),
+                                           _c_size_p, _c_double_p, _c_double_p]),
     "plk_profiling_enable": (ctypes.c_int, [ctypes.c_int]),
     "plk_profiling_get": (ctypes.c_int, [ctypes.c_char_p, _c_double_p, _c_double_p, _c_double_p]),
     "plk_profiling_names": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
@@ -363,6 +367,40 @@ def adjoint_jacobian(state: State, observables: List[Obs], ops: Ops, trainable: 
     check(load().plk_adjoint_jacobian(state.h, arr, nobs, ops.h, _sp(tp), tp.size,
                                       int(bool(apply_operations)), _dp(jac)))
     return jac[: nobs * tp.size].reshape(nobs, tp.size)
+
+
+def adjoint_batched(num_qubits: Sequence[int], ops: Sequence[Ops], observables: Sequence[Obs],
+                    trainable: Sequence[Sequence[int]] = None, dtype=np.complex64,
+                    device_id: int = 0):
+    """Expectation values and Jacobian rows of independent circuits run from
+    |0...0> in one batched launch. Returns (expvals, [row per circuit])."""
+    nc = len(num_qubits)
+    if len(ops) != nc or len(observables) != nc:
+        raise ValueError("num_qubits, ops and observables must have the same length")
+    code = PLK_C128 if np.dtype(dtype) == np.complex128 else PLK_C64
+    nq = (ctypes.c_int * max(1, nc))(*[int(q) for q in num_qubits])
+    oarr = (_vp * max(1, nc))(*[o.h.value for o in ops])
+    barr = (_vp * max(1, nc))(*[b.h.value for b in observables])
+    if trainable is None:
+        lens = [o.num_par_ops() for o in ops]
+        tarr = None
+        ntr = None
+        keep = []
+    else:
+        keep = [_sizes(t) for t in trainable]
+        lens = [k.size for k in keep]
+        tarr = (_c_size_p * max(1, nc))(*[_sp(k) for k in keep])
+        ntr = _sizes(lens)
+    total = int(sum(lens))
+    expvals = np.zeros(max(1, nc), dtype=np.float64)
+    jac = np.zeros(max(1, total), dtype=np.float64)
+    check(load().plk_adjoint_batched(code, int(device_id), nc, nq, oarr, barr, tarr,
+                                     None if ntr is None else _sp(ntr), _dp(expvals), _dp(jac)))
+    rows, off = [], 0
+    for L in lens:
+        rows.append(jac[off:off + L].copy())
+        off += L
+    return expvals[:nc], rows
 
 
 def profiling_enable(on: bool = True):

@@ -195,3 +195,29 @@ def c3_workload(n=24, layers=10, seed=20261204, ham_terms=32):
     obs = [("named", "PauliZ", [w]) for w in range(min(8, n))]
     obs.append(random_hamiltonian(n, ham_terms, rng))
     return n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs
+
+
+def c4_workload(count=4096, seed=20261205, nmin=6, nmax=16, dmin=2, dmax=8):
+    """Independent circuits: n uniform in [nmin, nmax], depth uniform in
+    [dmin, dmax] layers of RY on every wire, RZ on every wire, then the CNOT
+    chain (w, w + 1); parameters uniform in [0, 2 pi). Observable sum_i Z_i.
+    Returns a list of (n, ops, obs)."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(count):
+        n = int(rng.integers(nmin, nmax + 1))
+        depth = int(rng.integers(dmin, dmax + 1))
+        names, params, wires = [], [], []
+        for _layer in range(depth):
+            for g in ("RY", "RZ"):
+                for w in range(n):
+                    names.append(g)
+                    params.append([float(rng.uniform(0, 2 * np.pi))])
+                    wires.append([w])
+            for w in range(n - 1):
+                names.append("CNOT")
+                params.append([])
+                wires.append([w, w + 1])
+        obs = ("ham", [1.0] * n, [("named", "PauliZ", [w]) for w in range(n)])
+        out.append((n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs))
+    return out
