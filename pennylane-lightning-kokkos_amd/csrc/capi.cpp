@@ -422,10 +422,10 @@ int tile_bits_forward(int n, int dtype) {
 // Largest observable chunk whose tiles fit the LDS budget.
 int tile_max_nj(int dtype, int T) {
     int best = 1;
+    const int cap = env_int("PLK_TILE_NJ", dtype == PLK_C64 ? 6 : 4);
     for (int nj = 1; nj <= 8; nj++) {
         if (tile_nj_supported(dtype, nj) &&
-            tile_lds_bytes(dtype, T, nj, TILE_PMAX) <= TILE_LDS_MAX &&
-            nj <= (dtype == PLK_C64 ? 6 : 4)) {
+            tile_lds_bytes(dtype, T, nj, TILE_PMAX) <= TILE_LDS_MAX && nj <= cap) {
             best = nj;
         }
     }
@@ -444,7 +444,7 @@ std::vector<int> chunk_sizes(size_t n_obs, int njmax) {
 
 struct SegMeta {
     SegLaunch L{};
-    size_t first = 0; // index of the first SegOp in the uploaded table
+    size_t first = 0; // byte offset of the segment's op group in the table
     int cols[SEG_PMAX] = {0};
     double coefs[SEG_PMAX] = {0};
     TileLaunch TL{};
@@ -462,6 +462,64 @@ void put_mat(T *dst, const std::vector<cd> &M, bool swap_bits) {
             dst[2 * (r * dim + c) + 1] = static_cast<T>(v.imag());
         }
     }
+}
+
+template <typename T>
+size_t pack_ops(std::vector<unsigned char> &out, const SegOp<T> *ops, size_t nops) {
+    auto align16 = [&]() { out.resize((out.size() + 15) & ~size_t{15}, 0); };
+    align16();
+    const size_t base = out.size();
+    out.resize(base + nops * sizeof(OpHead), 0);
+    for (size_t o = 0; o < nops; o++) {
+        const SegOp<T> &d = ops[o];
+        const int dim = d.m == 1 ? 2 : 4;
+        std::vector<T> u, g;
+        switch (d.ukind) {
+        case U_DIAG:
+            for (int r = 0; r < dim; r++) {
+                u.push_back(d.U[2 * (r * dim + r)]);
+                u.push_back(d.U[2 * (r * dim + r) + 1]);
+            }
+            break;
+        case U_REAL:
+            for (int e = 0; e < dim * dim; e++) {
+                u.push_back(d.U[2 * e]);
+            }
+            break;
+        case U_SWAP:
+            break;
+        default:
+            u.assign(d.U, d.U + 2 * dim * dim);
+        }
+        if (d.slot >= 0) {
+            if (d.gkind == G_DIAG) {
+                for (int r = 0; r < dim; r++) {
+                    g.push_back(d.G[2 * (r * dim + r)]);
+                }
+            } else if (d.gkind == G_DENSE) {
+                g.assign(d.G, d.G + 2 * dim * dim);
+            }
+        }
+        align16();
+        const size_t uoff = out.size() - base;
+        out.insert(out.end(), reinterpret_cast<const unsigned char *>(u.data()),
+                   reinterpret_cast<const unsigned char *>(u.data() + u.size()));
+        align16();
+        const size_t goff = out.size() - base;
+        out.insert(out.end(), reinterpret_cast<const unsigned char *>(g.data()),
+                   reinterpret_cast<const unsigned char *>(g.data() + g.size()));
+        if (goff > 0xffff) {
+            raise(PLK_ERR_UNSUPPORTED, "operation group exceeds 64 KiB");
+        }
+        OpHead h{};
+        h.cls = d.cls;
+        h.slot = d.slot;
+        h.kinds = d.ukind | (d.gkind << 8) | (d.swap << 16);
+        h.offs = static_cast<int32_t>(uoff | (goff << 16));
+        std::memcpy(out.data() + base + o * sizeof(OpHead), &h, sizeof(OpHead));
+    }
+    align16();
+    return base;
 }
 
 // Picks the structure-specialized kernel path for a descriptor whose U (and
@@ -523,7 +581,13 @@ template <typename T> void classify(SegOp<T> &d, size_t m) {
                 }
             }
         }
-        if (gdiag) {
+        bool gz = gdiag;
+        for (int r = 0; r < dim && gz; r++) {
+            gz = re(d.G, r, r) == ((__builtin_popcount(r) & 1) ? -1 : 1);
+        }
+        if (gz) {
+            d.gkind = G_Z;
+        } else if (gdiag) {
             d.gkind = G_DIAG;
         } else if (m == 1 && re(d.G, 0, 0) == 0 && im(d.G, 0, 0) == 0 && re(d.G, 1, 1) == 0 &&
                    im(d.G, 1, 1) == 0) {
@@ -538,7 +602,7 @@ template <typename T> void classify(SegOp<T> &d, size_t m) {
 }
 
 template <typename T>
-SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
+SegMeta build_segment(const Segment &seg, int n, std::vector<unsigned char> &table) {
     SegMeta meta;
     SegLaunch &L = meta.L;
     L.n = n;
@@ -553,11 +617,12 @@ SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
         L.bits[q] = bw[q].first;
         pos[bw[q].second] = q;
     }
-    meta.first = table.size();
+    std::vector<SegOp<T>> group;
     int slot = 0;
     for (const auto &p : seg.ops) {
         SegOp<T> d{};
         d.slot = -1;
+        d.m = static_cast<int32_t>(p.U.wires.size());
         const auto &w = p.U.wires;
         bool swapb = false;
         if (w.size() == 1) {
@@ -586,15 +651,16 @@ SegMeta build_segment(const Segment &seg, int n, std::vector<SegOp<T>> &table) {
             slot++;
         }
         classify(d, w.size());
-        table.push_back(d);
+        group.push_back(d);
     }
+    meta.first = pack_ops(table, group.data(), group.size());
     L.nops = static_cast<int>(seg.ops.size());
     L.nslots = slot;
     return meta;
 }
 
 template <typename T>
-void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
+void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
                 std::vector<TileBundle> &btab, SegMeta &meta) {
     TileLaunch &TL = meta.TL;
     TL.n = n;
@@ -613,14 +679,15 @@ void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
         tb.K = b.K;
         tb.qa = b.qa;
         tb.qb = b.qb;
-        tb.first = static_cast<int32_t>(table.size());
         tb.nops = static_cast<int32_t>(b.ops.size());
         tb.nslots = b.nslots;
         tb.slot0 = slot0;
         int slot = 0;
+        std::vector<SegOp<T>> group;
         for (const auto &p : b.ops) {
             SegOp<T> d{};
             d.slot = -1;
+            d.m = static_cast<int32_t>(p.U.wires.size());
             const auto &w = p.U.wires;
             bool swapb = false;
             if (w.size() == 1) {
@@ -637,8 +704,13 @@ void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
                 d.slot = slot++;
             }
             classify(d, w.size());
-            table.push_back(d);
+            group.push_back(d);
         }
+        const size_t off = pack_ops(table, group.data(), group.size());
+        if (off > static_cast<size_t>(INT32_MAX)) {
+            raise(PLK_ERR_UNSUPPORTED, "operation table exceeds 2 GiB");
+        }
+        tb.first = static_cast<int32_t>(off);
         slot0 += b.nslots;
         btab.push_back(tb);
     }
@@ -648,8 +720,7 @@ void build_pass(const Pass &ps, int n, std::vector<SegOp<T>> &table,
 
 // Device tables of segment/pass descriptors for a whole plan (one upload).
 struct SegTable {
-    void *d = nullptr;
-    size_t elem = 0;
+    void *d = nullptr;       // packed op groups
     void *bundles = nullptr;
     ~SegTable() {
         if (d || bundles) {
@@ -666,7 +737,7 @@ struct SegTable {
 
 template <typename T>
 std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable &tab) {
-    std::vector<SegOp<T>> table;
+    std::vector<unsigned char> table;
     std::vector<TileBundle> btab;
     std::vector<SegMeta> metas;
     for (const auto &s : steps) {
@@ -680,11 +751,9 @@ std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable
             metas.push_back(SegMeta{});
         }
     }
-    tab.elem = sizeof(SegOp<T>);
     if (!table.empty()) {
-        PLK_HIP(hipMalloc(&tab.d, table.size() * sizeof(SegOp<T>)));
-        PLK_HIP(hipMemcpy(tab.d, table.data(), table.size() * sizeof(SegOp<T>),
-                          hipMemcpyHostToDevice));
+        PLK_HIP(hipMalloc(&tab.d, table.size()));
+        PLK_HIP(hipMemcpy(tab.d, table.data(), table.size(), hipMemcpyHostToDevice));
     }
     if (!btab.empty()) {
         PLK_HIP(hipMalloc(&tab.bundles, btab.size() * sizeof(TileBundle)));
@@ -712,7 +781,7 @@ void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
             L.write_phi = 1;
             L.nj = 0;
             L.nslots = 0;
-            const void *ops = static_cast<const char *>(tab.d) + metas[i].first * tab.elem;
+            const void *ops = static_cast<const char *>(tab.d) + metas[i].first;
             launch_segment(dev, s, nullptr, ops, L);
         } else if (steps[i].tiled) {
             TileLaunch TL = metas[i].TL;
@@ -965,7 +1034,7 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
         const Step &st = steps[i];
         if (st.fused) {
             const SegMeta &m = metas[i];
-            const void *d_ops = static_cast<const char *>(tab.d) + m.first * tab.elem;
+            const void *d_ops = static_cast<const char *>(tab.d) + m.first;
             const int njmax = std::min(kmax_nj, seg_max_nj(ref.dtype, m.L.K));
             size_t j0 = 0;
             while (j0 < n_obs) {
@@ -1056,7 +1125,7 @@ template <typename T> struct BatchPlan {
     std::vector<BatchCircuit> circuits;
     std::vector<BatchPass> passes;
     std::vector<TileBundle> bundles;
-    std::vector<SegOp<T>> ops;
+    std::vector<unsigned char> ops; // packed op groups
     std::vector<BatchTerm> terms;
     std::vector<int32_t> slot_cols;
     std::vector<double> slot_coefs;

@@ -228,9 +228,6 @@ __device__ __forceinline__ void dispatch_cls(int cls, F &&f) {
     }
 }
 
-template <typename T>
-using ConstSegOp = __attribute__((address_space(4))) const SegOp<T>;
-
 template <typename T> struct SegPtrs {
     Cx<T> *p[SEG_NJMAX];
 };
@@ -276,7 +273,7 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
 #pragma unroll
         for (int i = 0; i < D; i++) {
             const int r = op_index<M, QA, QB>(i);
-            v[i] = cmul(Cx<T>{U[2 * (r * dim + r)], U[2 * (r * dim + r) + 1]}, v[i]);
+            v[i] = cmul(Cx<T>{U[2 * r], U[2 * r + 1]}, v[i]);
         }
     } else if (ukind == U_SWAP) {
         if constexpr (M == 1) {
@@ -293,7 +290,7 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
         }
     } else if (ukind == U_REAL) {
         if constexpr (M == 1) {
-            const T m00 = U[0], m01 = U[2], m10 = U[4], m11 = U[6];
+            const T m00 = U[0], m01 = U[1], m10 = U[2], m11 = U[3];
 #pragma unroll
             for (int i = 0; i < D; i++) {
                 if (i & (1 << QA)) {
@@ -318,11 +315,11 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
                 }
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    T sr = U[2 * (4 * r)] * a[0].r, si = U[2 * (4 * r)] * a[0].i;
+                    T sr = U[4 * r] * a[0].r, si = U[4 * r] * a[0].i;
 #pragma unroll
                     for (int c = 1; c < 4; c++) {
-                        sr = fma(U[2 * (4 * r + c)], a[c].r, sr);
-                        si = fma(U[2 * (4 * r + c)], a[c].i, si);
+                        sr = fma(U[4 * r + c], a[c].r, sr);
+                        si = fma(U[4 * r + c], a[c].i, si);
                     }
                     v[idx[r]] = {sr, si};
                 }
@@ -353,11 +350,17 @@ __device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1
                                          const __attribute__((address_space(4))) T *G) {
     constexpr int D = 1 << K;
     constexpr int dim = 1 << M;
-    if (gkind == G_DIAG) {
+    if (gkind == G_Z) {
 #pragma unroll
         for (int i = 0; i < D; i++) {
             const int r = op_index<M, QA, QB>(i);
-            const T g = G[2 * (r * dim + r)];
+            gf[i] = (__builtin_popcount(r) & 1) ? Cx<T>{-f[i].r, -f[i].i} : f[i];
+        }
+    } else if (gkind == G_DIAG) {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            const int r = op_index<M, QA, QB>(i);
+            const T g = G[r];
             gf[i] = {g * f[i].r, g * f[i].i};
         }
     } else if (M == 1 && gkind == G_X) {
@@ -399,16 +402,21 @@ __device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? NJ : 1][1 << K],
                                         double (&acc)[SEG_PMAX][NJ > 0 ? NJ : 1],
-                                        const SegOp<T> *ops, int nops) {
+                                        const void *group, int nops) {
     constexpr int D = 1 << K;
+    using CT = __attribute__((address_space(4))) const T;
+    const auto *base = (const __attribute__((address_space(4))) unsigned char *)(group);
+    const auto *heads = reinterpret_cast<const __attribute__((address_space(4))) OpHead *>(base);
     for (int o = 0; o < nops; o++) {
-        const ConstSegOp<T> &op = reinterpret_cast<const ConstSegOp<T> *>(
-            (const __attribute__((address_space(4))) char *)(ops))[o];
-        const int cls = op.cls;
-        const int slot = op.slot;
-        const int ukind = op.ukind;
-        const int gkind = op.gkind;
-        const int swp = op.swap;
+        const int cls = heads[o].cls;
+        const int slot = heads[o].slot;
+        const int kinds = heads[o].kinds;
+        const uint32_t offs = static_cast<uint32_t>(heads[o].offs);
+        const int ukind = kinds & 255;
+        const int gkind = (kinds >> 8) & 255;
+        const int swp = kinds >> 16;
+        CT *U = reinterpret_cast<CT *>(base + (offs & 0xffff));
+        CT *G = reinterpret_cast<CT *>(base + (offs >> 16));
         dispatch_cls<K>(cls, [&](auto Mc, auto QAc, auto QBc) {
             constexpr int m = decltype(Mc)::value;
             constexpr int qa = decltype(QAc)::value;
@@ -416,7 +424,7 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
             if constexpr (NJ > 0) {
                 if (slot >= 0) {
                     Cx<T> gf[D];
-                    gen_kind<T, K, m, qa, qb>(gf, f, gkind, op.G);
+                    gen_kind<T, K, m, qa, qb>(gf, f, gkind, G);
 #pragma unroll
                     for (int j = 0; j < NJ; j++) {
                         const double d = im_dot(h[j], gf);
@@ -429,11 +437,11 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
                     }
                 }
             }
-            apply_kind<T, K, m, qa, qb>(f, ukind, swp, op.U);
+            apply_kind<T, K, m, qa, qb>(f, ukind, swp, U);
             if constexpr (NJ > 0) {
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
-                    apply_kind<T, K, m, qa, qb>(h[j], ukind, swp, op.U);
+                    apply_kind<T, K, m, qa, qb>(h[j], ukind, swp, U);
                 }
             }
         });

@@ -20,7 +20,7 @@ namespace plk {
 template <typename T, int K, int NJ, typename Off>
 __global__ void __launch_bounds__(SEG_BLOCK)
     seg_kernel(Cx<T> *__restrict__ phi, SegPtrs<T> hp,
-               const SegOp<T> *__restrict__ ops, SegLaunch L, uint64_t ngroups,
+               const unsigned char *__restrict__ ops, SegLaunch L, uint64_t ngroups,
                double *__restrict__ partials) {
     constexpr int D = 1 << K;
     constexpr int NA = NJ > 0 ? NJ : 1;
@@ -275,11 +275,11 @@ int seg_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void 
     if (phi.bytes() <= (uint64_t{1} << 32)) {
         hipLaunchKernelGGL((seg_kernel<T, K, NJ, uint32_t>), dim3(grid), dim3(SEG_BLOCK), 0,
                            dev.stream, static_cast<Cx<T> *>(phi.d), hp,
-                           static_cast<const SegOp<T> *>(d_ops), L, ng, dev.red);
+                           static_cast<const unsigned char *>(d_ops), L, ng, dev.red);
     } else {
         hipLaunchKernelGGL((seg_kernel<T, K, NJ, uint64_t>), dim3(grid), dim3(SEG_BLOCK), 0,
                            dev.stream, static_cast<Cx<T> *>(phi.d), hp,
-                           static_cast<const SegOp<T> *>(d_ops), L, ng, dev.red);
+                           static_cast<const unsigned char *>(d_ops), L, ng, dev.red);
     }
     PLK_HIP(hipGetLastError());
     return static_cast<int>(grid);
@@ -513,9 +513,5 @@ void launch_axpy(Device &dev, const State &y, double alpha, const State &x) {
     }
     PLK_HIP(hipGetLastError());
 }
-
-// Explicit instantiation of the segment operand layout for the host side.
-template struct SegOp<float>;
-template struct SegOp<double>;
 
 } // namespace plk

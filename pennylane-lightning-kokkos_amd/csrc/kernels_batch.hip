@@ -43,7 +43,7 @@ __device__ __forceinline__ uint32_t deposit(uint32_t t, uint32_t mask) {
 // (tile positions qa, qb) goes through the bundle's ops in registers.
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void bundle_tile(Cx<T> *phi, Cx<T> *h, int Tb, const TileBundle &bd,
-                                            const SegOp<T> *ops,
+                                            const unsigned char *ops,
                                             double (&acc)[SEG_PMAX][NJ > 0 ? NJ : 1]) {
     constexpr int D = 1 << K;
     uint32_t o[D];
@@ -96,7 +96,7 @@ struct BatchShared {
 template <typename T, int NJ>
 __device__ __forceinline__ void run_bundles(const BatchArgs &A, const BatchPass &P, Cx<T> *lphi,
                                             Cx<T> *lh, BatchShared &sh, int &parity) {
-    const SegOp<T> *ops = static_cast<const SegOp<T> *>(A.ops);
+    const unsigned char *ops = static_cast<const unsigned char *>(A.ops);
     const int tid = threadIdx.x;
     for (int b = 0; b < P.bundle_count; b++) {
         const TileBundle bd = A.bundles[P.bundle_first + b];

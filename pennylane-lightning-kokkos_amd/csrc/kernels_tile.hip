@@ -26,7 +26,7 @@ __device__ __forceinline__ uint32_t ins0(uint32_t k, int b) {
 
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void run_bundle(Cx<T> *__restrict__ tile, int TS,
-                                           const TileBundle &bd, const SegOp<T> *ops,
+                                           const TileBundle &bd, const unsigned char *ops,
                                            double *__restrict__ lacc, int pass_slots) {
     constexpr int D = 1 << K;
     constexpr int NA = NJ > 0 ? NJ : 1;
@@ -107,7 +107,7 @@ __device__ __forceinline__ void run_bundle(Cx<T> *__restrict__ tile, int TS,
 template <typename T, int NJ, typename Off>
 __global__ void __launch_bounds__(TILE_BLOCK)
     tile_kernel(Cx<T> *__restrict__ phi, SegPtrs<T> hp, const TileBundle *__restrict__ bundles,
-                const SegOp<T> *__restrict__ ops, TileLaunch L, double *__restrict__ partials) {
+                const unsigned char *__restrict__ ops, TileLaunch L, double *__restrict__ partials) {
     extern __shared__ __align__(16) unsigned char smem[];
     constexpr int NS = NJ + 1;
     const int TS = 1 << L.T;
@@ -249,13 +249,13 @@ int tile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void
         hipLaunchKernelGGL((tile_kernel<T, NJ, uint32_t>), dim3(grid), dim3(TILE_BLOCK), lds,
                            dev.stream, static_cast<Cx<T> *>(phi.d), hp,
                            static_cast<const TileBundle *>(d_bundles),
-                           static_cast<const SegOp<T> *>(d_ops), L, dev.red);
+                           static_cast<const unsigned char *>(d_ops), L, dev.red);
     } else {
         set_lds_limit<T, NJ, uint64_t>();
         hipLaunchKernelGGL((tile_kernel<T, NJ, uint64_t>), dim3(grid), dim3(TILE_BLOCK), lds,
                            dev.stream, static_cast<Cx<T> *>(phi.d), hp,
                            static_cast<const TileBundle *>(d_bundles),
-                           static_cast<const SegOp<T> *>(d_ops), L, dev.red);
+                           static_cast<const unsigned char *>(d_ops), L, dev.red);
     }
     PLK_HIP(hipGetLastError());
     return static_cast<int>(grid);

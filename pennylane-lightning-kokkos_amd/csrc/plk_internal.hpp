@@ -134,25 +134,40 @@ constexpr int SEG_PMAX = 4;      // trainable ops per segment
 constexpr int SEG_NJMAX = 8;     // observable states per launch
 constexpr int SEG_BLOCK = 256;
 
-// One fused operation as read by the segment kernel. `cls` selects the
-// compile-time local-position case; U and G are row-major complex matrices
-// in the state's precision (4x4 for two-wire ops, 2x2 in the first 8 reals
-// for one-wire ops).
 // Structure of an operation's matrix, chosen on the host so the kernels skip
 // the multiplications a dense complex product would waste.
 enum SegUKind : int32_t { U_DENSE = 0, U_DIAG = 1, U_REAL = 2, U_SWAP = 3 };
-// Structure of a generator: dense, real diagonal, or (one wire) X or Y.
-enum SegGKind : int32_t { G_DENSE = 0, G_DIAG = 1, G_X = 2, G_Y = 3 };
+// Structure of a generator: dense, real diagonal, (one wire) X or Y, or the
+// parity sign (-1)^popcount of the op-local index (Z, Z(x)Z).
+enum SegGKind : int32_t { G_DENSE = 0, G_DIAG = 1, G_X = 2, G_Y = 3, G_Z = 4 };
 
+// Host-side description of one fused operation: `cls` selects the
+// compile-time local-position case, m is 1 or 2 wires, U and G are row-major
+// complex matrices (4x4, or 2x2 in the first 8 reals). Packed for the device
+// on the host into an op group (see OpHead).
 template <typename T> struct SegOp {
     int32_t cls;
     int32_t slot;  // trainable slot in [0, SEG_PMAX) or -1
     int32_t ukind; // SegUKind
     int32_t gkind; // SegGKind
     int32_t swap;  // U_SWAP: 4 * a + b, local indices a < b exchanged
-    int32_t pad[3];
+    int32_t m;     // wires of the op
+    int32_t pad[2];
     T U[32];
     T G[32];
+};
+
+// Device op group (a segment or a bundle): nops 16-byte headers, then the
+// payloads. A header's offsets are bytes from the group start; the payloads
+// hold only what the kind reads:
+//   U_DENSE 2 dim^2 reals (row-major complex), U_DIAG 2 dim (complex
+//   diagonal), U_REAL dim^2 (row-major real), U_SWAP none;
+//   G_DENSE 2 dim^2, G_DIAG dim (real diagonal), G_X / G_Y / G_Z none.
+struct OpHead {
+    int32_t cls;
+    int32_t slot;
+    int32_t kinds; // ukind | gkind << 8 | swap << 16
+    int32_t offs;  // U offset | G offset << 16
 };
 
 struct SegLaunch {
@@ -255,7 +270,7 @@ struct BatchArgs {
     const int64_t *block_scratch; // amplitude offset of each workgroup's slice
     const BatchPass *passes;
     const TileBundle *bundles;
-    const void *ops;             // SegOp<T>
+    const void *ops;             // packed op groups (OpHead)
     const BatchTerm *terms;
     const int32_t *slot_cols;
     const double *slot_coefs;
