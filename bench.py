@@ -108,6 +108,66 @@ def cpu_baseline(n, ops, obs, sample_params):
                        f"{dt:.1f} s")}
 
 
+def run_batched(capi, count, seq_count):
+    """The 4096-circuit batched workload: n uniform in [6, 16], 2-8 layers of
+    RY and RZ on every wire plus a CNOT chain, observable sum_i Z_i, every
+    parameter trainable. Timed: one plk_adjoint_batched call in complex64
+    (FP64 accumulation) and in complex128, each after one untimed call, and
+    the plain per-circuit complex128 path (plk_adjoint_jacobian with
+    apply_operations, states preallocated per qubit count) over the first
+    seq_count circuits. Handles are built before the clocks start."""
+    from helpers import c4_workload
+
+    circuits = c4_workload(count=count)
+    nq = [c[0] for c in circuits]
+    ops = [capi.Ops(*c[1]) for c in circuits]
+    obs = [capi.Obs.hamiltonian(np.ones(c[0]), [capi.Obs.named("PauliZ", [w]) for w in range(c[0])])
+           for c in circuits]
+    params = sum(sum(1 for p in c[1][1] if p) for c in circuits)
+    out = {"workload": (f"{count} circuits, n uniform in [6,16], 2-8 layers of RY and RZ on every "
+                        f"wire then CNOT chain, observable sum_i Z_i, all parameters trainable"),
+           "circuits": count, "params": params}
+    rows = {}
+    for name, dt in (("complex64", np.complex64), ("complex128", np.complex128)):
+        capi.adjoint_batched(nq, ops, obs, None, dtype=dt)
+        capi.profiling_enable(True)
+        t0 = time.perf_counter()
+        ev, r = capi.adjoint_batched(nq, ops, obs, None, dtype=dt)
+        dt_call = time.perf_counter() - t0
+        k = capi.profiling_get("adjoint_batch")
+        capi.profiling_enable(False)
+        rows[name] = (ev, r)
+        out[name] = {"call_ms": round(1e3 * dt_call, 2), "kernel_ms": round(k["total_ms"], 3),
+                     "params_per_s_call": round(params / dt_call, 1),
+                     "params_per_s_kernel": round(params / (k["total_ms"] * 1e-3), 1)}
+        log(f"bench: batched {name}: call {1e3 * dt_call:.1f} ms, kernel {k['total_ms']:.2f} ms")
+    states = {}
+    seq = circuits[:seq_count]
+    seq_params = sum(sum(1 for p in c[1][1] if p) for c in seq)
+    for n in sorted(set(c[0] for c in seq)):
+        states[n] = capi.State(n, np.complex128)
+    jacs = []
+    t0 = time.perf_counter()
+    for i, c in enumerate(seq):
+        s = states[c[0]]
+        s.reset()
+        npar = sum(1 for p in c[1][1] if p)
+        jacs.append(capi.adjoint_jacobian(s, [obs[i]], ops[i], list(range(npar)), True)[0])
+    dt_seq = time.perf_counter() - t0
+    out["sequential_complex128"] = {"circuits": len(seq), "ms": round(1e3 * dt_seq, 1),
+                                    "params_per_s": round(seq_params / dt_seq, 1)}
+    ev64, r64 = rows["complex64"]
+    ev128, r128 = rows["complex128"]
+    out["max_abs_err_complex64_vs_per_circuit_complex128"] = float(
+        max(np.max(np.abs(r64[i] - jacs[i])) for i in range(len(seq))))
+    out["max_abs_err_complex128_vs_per_circuit_complex128"] = float(
+        max(np.max(np.abs(r128[i] - jacs[i])) for i in range(len(seq))))
+    out["speedup_complex64_call_vs_sequential"] = round(
+        out["complex64"]["params_per_s_call"] / out["sequential_complex128"]["params_per_s"], 1)
+    log(f"bench: sequential per-circuit complex128: {len(seq)} circuits {dt_seq:.2f} s")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -117,6 +177,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--c64-steps", type=int, default=3)
+    ap.add_argument("--batched", type=int, default=4096,
+                    help="circuits of the batched workload (0 skips it)")
+    ap.add_argument("--batched-seq", type=int, default=4096,
+                    help="circuits timed through the per-circuit path for comparison")
     ap.add_argument("--probe-copy", action="store_true",
                     help="only copy one 24-qubit complex128 state to another 10 times "
                          "(256 MiB read + 256 MiB written per copy): the calibration "
@@ -171,6 +235,10 @@ def main():
         extra["complex64"] = {"params_per_s": round(ntp * args.c64_steps / dt64, 1),
                               "ms_per_step": round(1e3 * dt64 / args.c64_steps, 2),
                               "max_abs_diff_vs_complex128": float(np.max(np.abs(jac64 - jac)))}
+
+    if args.batched > 0:
+        log(f"bench: batched workload ({args.batched} circuits)")
+        extra["batched"] = run_batched(capi, args.batched, min(args.batched_seq, args.batched))
 
     cpu = None
     if not args.no_cpu:

@@ -9,6 +9,7 @@
 #include <queue>
 #include <set>
 #include <sstream>
+#include <thread>
 
 #include "plk_internal.hpp"
 
@@ -274,42 +275,38 @@ constexpr size_t kLookahead = 512;
 
 // Splits a pass's ops (in order) into bundles of <= 2 tile bits.
 void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
-    std::map<int, int> pos;
+    int pos[64];
     for (size_t q = 0; q < ps.tb.size(); q++) {
         pos[ps.tb[q]] = static_cast<int>(q);
     }
     Bundle cur;
-    std::set<int> cb; // tile positions of the current bundle
+    uint32_t cb = 0; // tile positions of the current bundle
     auto flush = [&]() {
         if (!cur.ops.empty()) {
-            std::vector<int> v(cb.begin(), cb.end());
-            if (v.size() == 1) {
+            const int lo = __builtin_ctz(cb);
+            if (__builtin_popcount(cb) == 1) {
                 cur.K = 1;
-                cur.qa = v[0];
+                cur.qa = lo;
             } else {
                 cur.K = 2;
-                cur.qa = v[1];
-                cur.qb = v[0];
+                cur.qa = 31 - __builtin_clz(cb);
+                cur.qb = lo;
             }
             ps.bundles.push_back(std::move(cur));
         }
         cur = Bundle{};
-        cb.clear();
+        cb = 0;
     };
     for (auto &p : ops) {
-        std::set<int> nb = cb;
-        for (int b : op_bits(p, n)) {
-            nb.insert(pos.at(b));
+        uint32_t ob = 0;
+        for (auto w : p.U.wires) {
+            ob |= 1u << pos[n - 1 - static_cast<int>(w)];
         }
-        if (nb.size() > 2 || cur.ops.size() >= kSegMaxOps ||
+        if (__builtin_popcount(cb | ob) > 2 || cur.ops.size() >= kSegMaxOps ||
             (p.trainable && cur.nslots >= SEG_PMAX)) {
             flush();
-            nb.clear();
-            for (int b : op_bits(p, n)) {
-                nb.insert(pos.at(b));
-            }
         }
-        cb = nb;
+        cb |= ob;
         if (p.trainable) {
             cur.nslots++;
             ps.cols.push_back(p.col);
@@ -345,11 +342,9 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
             head++;
             continue;
         }
-        std::set<int> bits;
-        for (int b = 0; b < L; b++) {
-            bits.insert(b);
-        }
-        std::set<size_t> blocked;
+        // Tile bits and blocked wires as masks (state bits and wires < 64).
+        uint64_t bits = L > 0 ? (uint64_t{1} << L) - 1 : 0;
+        uint64_t blocked = 0;
         std::vector<PlanOp> taken;
         std::vector<PlanOp> rest;
         int nslots = 0;
@@ -359,21 +354,19 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
             if (!fusable(p) || k - head >= kLookahead) {
                 break;
             }
-            bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < pmax);
+            uint64_t wmask = 0, nb = bits;
             for (auto w : p.U.wires) {
-                ok = ok && !blocked.count(w);
+                wmask |= uint64_t{1} << w;
+                nb |= uint64_t{1} << (n - 1 - static_cast<int>(w));
             }
-            std::set<int> nb = bits;
-            for (int b : op_bits(p, n)) {
-                nb.insert(b);
-            }
-            ok = ok && nb.size() <= static_cast<size_t>(T);
+            const bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < pmax) &&
+                            !(blocked & wmask) && __builtin_popcountll(nb) <= T;
             if (ok) {
                 bits = nb;
                 nslots += p.trainable ? 1 : 0;
                 taken.push_back(std::move(p));
             } else {
-                blocked.insert(p.U.wires.begin(), p.U.wires.end());
+                blocked |= wmask;
                 rest.push_back(std::move(p));
             }
         }
@@ -381,12 +374,16 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
             rest.push_back(std::move(rem[k]));
         }
         // Pad the tile with the lowest unused bits.
-        for (int b = 0; static_cast<int>(bits.size()) < T && b < n; b++) {
-            bits.insert(b);
+        for (int b = 0; __builtin_popcountll(bits) < T && b < n; b++) {
+            bits |= uint64_t{1} << b;
         }
         Step s;
         s.tiled = true;
-        s.pass.tb.assign(bits.begin(), bits.end());
+        for (int b = 0; b < 64; b++) {
+            if (bits >> b & 1) {
+                s.pass.tb.push_back(b);
+            }
+        }
         make_bundles(s.pass, std::move(taken), n);
         steps.push_back(std::move(s));
         rem = std::move(rest);
@@ -473,41 +470,46 @@ size_t pack_ops(std::vector<unsigned char> &out, const SegOp<T> *ops, size_t nop
     for (size_t o = 0; o < nops; o++) {
         const SegOp<T> &d = ops[o];
         const int dim = d.m == 1 ? 2 : 4;
-        std::vector<T> u, g;
+        T u[32], g[32];
+        int nu = 0, ng = 0;
         switch (d.ukind) {
         case U_DIAG:
             for (int r = 0; r < dim; r++) {
-                u.push_back(d.U[2 * (r * dim + r)]);
-                u.push_back(d.U[2 * (r * dim + r) + 1]);
+                u[nu++] = d.U[2 * (r * dim + r)];
+                u[nu++] = d.U[2 * (r * dim + r) + 1];
             }
             break;
         case U_REAL:
             for (int e = 0; e < dim * dim; e++) {
-                u.push_back(d.U[2 * e]);
+                u[nu++] = d.U[2 * e];
             }
             break;
         case U_SWAP:
             break;
         default:
-            u.assign(d.U, d.U + 2 * dim * dim);
+            for (int e = 0; e < 2 * dim * dim; e++) {
+                u[nu++] = d.U[e];
+            }
         }
         if (d.slot >= 0) {
             if (d.gkind == G_DIAG) {
                 for (int r = 0; r < dim; r++) {
-                    g.push_back(d.G[2 * (r * dim + r)]);
+                    g[ng++] = d.G[2 * (r * dim + r)];
                 }
             } else if (d.gkind == G_DENSE) {
-                g.assign(d.G, d.G + 2 * dim * dim);
+                for (int e = 0; e < 2 * dim * dim; e++) {
+                    g[ng++] = d.G[e];
+                }
             }
         }
         align16();
         const size_t uoff = out.size() - base;
-        out.insert(out.end(), reinterpret_cast<const unsigned char *>(u.data()),
-                   reinterpret_cast<const unsigned char *>(u.data() + u.size()));
+        out.insert(out.end(), reinterpret_cast<const unsigned char *>(u),
+                   reinterpret_cast<const unsigned char *>(u + nu));
         align16();
         const size_t goff = out.size() - base;
-        out.insert(out.end(), reinterpret_cast<const unsigned char *>(g.data()),
-                   reinterpret_cast<const unsigned char *>(g.data() + g.size()));
+        out.insert(out.end(), reinterpret_cast<const unsigned char *>(g),
+                   reinterpret_cast<const unsigned char *>(g + ng));
         if (goff > 0xffff) {
             raise(PLK_ERR_UNSUPPORTED, "operation group exceeds 64 KiB");
         }
@@ -668,7 +670,7 @@ void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
     for (int q = 0; q < TL.T; q++) {
         TL.tb[q] = ps.tb[q];
     }
-    std::map<int, int> pos;
+    int pos[64];
     for (size_t q = 0; q < ps.tb.size(); q++) {
         pos[ps.tb[q]] = static_cast<int>(q);
     }
@@ -691,10 +693,10 @@ void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
             const auto &w = p.U.wires;
             bool swapb = false;
             if (w.size() == 1) {
-                const int q = pos.at(n - 1 - static_cast<int>(w[0]));
+                const int q = pos[n - 1 - static_cast<int>(w[0])];
                 d.cls = (b.K == 1 || q == b.qb) ? 0 : 1;
             } else {
-                const int q0 = pos.at(n - 1 - static_cast<int>(w[0]));
+                const int q0 = pos[n - 1 - static_cast<int>(w[0])];
                 swapb = (q0 != b.qa);
                 d.cls = 2;
             }
@@ -1238,21 +1240,90 @@ struct Blob {
     }
 };
 
+// Appends src's circuits to dst, rebasing every index and offset.
+template <typename T> void merge_plan(BatchPlan<T> &dst, BatchPlan<T> &&src) {
+    const int32_t pass_off = static_cast<int32_t>(dst.passes.size());
+    const int32_t bundle_off = static_cast<int32_t>(dst.bundles.size());
+    const int32_t term_off = static_cast<int32_t>(dst.terms.size());
+    const int32_t slot_off = static_cast<int32_t>(dst.slot_cols.size());
+    dst.ops.resize((dst.ops.size() + 15) & ~size_t{15}, 0);
+    const size_t ops_off = dst.ops.size();
+    if (ops_off + src.ops.size() > static_cast<size_t>(INT32_MAX)) {
+        raise(PLK_ERR_UNSUPPORTED, "operation table exceeds 2 GiB");
+    }
+    for (auto b : src.bundles) {
+        b.first += static_cast<int32_t>(ops_off);
+        dst.bundles.push_back(b);
+    }
+    for (auto p : src.passes) {
+        p.bundle_first += bundle_off;
+        p.slot_first += slot_off;
+        dst.passes.push_back(p);
+    }
+    for (auto c : src.circuits) {
+        c.fwd_first += pass_off;
+        c.bwd_first += pass_off;
+        c.term_first += term_off;
+        c.jac_offset += dst.jac_len;
+        dst.circuits.push_back(c);
+    }
+    dst.ops.insert(dst.ops.end(), src.ops.begin(), src.ops.end());
+    dst.terms.insert(dst.terms.end(), src.terms.begin(), src.terms.end());
+    dst.slot_cols.insert(dst.slot_cols.end(), src.slot_cols.begin(), src.slot_cols.end());
+    dst.slot_coefs.insert(dst.slot_coefs.end(), src.slot_coefs.begin(), src.slot_coefs.end());
+    dst.cost.insert(dst.cost.end(), src.cost.begin(), src.cost.end());
+    dst.jac_len += src.jac_len;
+}
+
 template <typename T>
 void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double *expvals,
                        double *jac) {
     const int dtype = sizeof(T) == 4 ? PLK_C64 : PLK_C128;
     const int tile_bits = batch_tile_bits(dtype);
-    BatchPlan<T> bp;
-    for (size_t i = 0; i < in.size(); i++) {
-        try {
-            plan_batch_circuit<T>(bp, in[i], tile_bits, low_bits(dtype));
-        } catch (const Error &e) {
-            raise(e.code, "circuit " + std::to_string(i) + ": " + e.msg);
-        }
-    }
     if (in.empty()) {
         return;
+    }
+    // Circuits are planned independently on host threads, in contiguous
+    // chunks, and the chunk plans are concatenated in circuit order.
+    const size_t hw = std::max(1u, std::thread::hardware_concurrency());
+    const size_t nthreads = std::min<size_t>(
+        {static_cast<size_t>(std::max(1, env_int("PLK_PLAN_THREADS", static_cast<int>(hw)))),
+         (in.size() + 31) / 32, size_t{64}});
+    std::vector<BatchPlan<T>> parts(nthreads);
+    std::vector<std::pair<size_t, Error>> failures(nthreads, {SIZE_MAX, Error{PLK_OK, ""}});
+    auto work = [&](size_t t) {
+        const size_t lo = in.size() * t / nthreads, hi = in.size() * (t + 1) / nthreads;
+        for (size_t i = lo; i < hi; i++) {
+            try {
+                plan_batch_circuit<T>(parts[t], in[i], tile_bits, low_bits(dtype));
+            } catch (const Error &e) {
+                failures[t] = {i, e};
+                return;
+            } catch (const std::exception &e) {
+                failures[t] = {i, Error{PLK_ERR_INVALID, e.what()}};
+                return;
+            }
+        }
+    };
+    if (nthreads == 1) {
+        work(0);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < nthreads; t++) {
+            pool.emplace_back(work, t);
+        }
+        for (auto &th : pool) {
+            th.join();
+        }
+    }
+    for (const auto &f : failures) {
+        if (f.first != SIZE_MAX) {
+            raise(f.second.code, "circuit " + std::to_string(f.first) + ": " + f.second.msg);
+        }
+    }
+    BatchPlan<T> bp = std::move(parts[0]);
+    for (size_t t = 1; t < nthreads; t++) {
+        merge_plan(bp, std::move(parts[t]));
     }
     set_device(device_id);
     Device &dev = device(device_id);
