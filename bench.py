@@ -31,11 +31,11 @@ for _p in (os.path.join(REPO, "pennylane-lightning-kokkos_amd"), os.path.join(RE
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_JSON = os.path.join(REPO, "profiles", "adjoint_segment_traffic.json")
+TRAFFIC_JSON = os.path.join(REPO, "profiles", "adjoint_traffic.json")
 
 
 def measured_traffic(family):
-    """HBM bytes per adjoint-segment launch measured by rocprofv3 --pmc
+    """HBM bytes per launch of the dominant adjoint kernel family measured by rocprofv3 --pmc
     (FETCH_SIZE and WRITE_SIZE, scaled by a copy kernel of known size), valid
     only while the kernel sources are the ones that were profiled."""
     import glob
@@ -228,7 +228,23 @@ def main():
                      "GBps": round(v["bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)}
                  for k, v in stats.items()}
 
-    extra = {}
+    # Sweep-level accounting of the unfused algorithm: per trainable rotation
+    # 2 S (1 + n_obs) bytes, per CNOT S (1 + n_obs) (every state read and
+    # written once per gate). Fused passes move far fewer bytes; this figure
+    # states what the same sweep would cost one gate per pass.
+    S = (2 ** n) * 16
+    n_rot = sum(1 for p in ops[1] if p)
+    n_other = len(ops[0]) - n_rot
+    unfused = (2 * n_rot + n_other) * S * (1 + len(obs))
+    adj_ms = sum(v["total_ms"] for k, v in stats.items()
+                 if k.startswith("adjoint_") or k.startswith("reduce")) / args.steps
+    passes = seg["launches"] / args.steps / max(1, -(-len(obs) // 4)) if seg["launches"] else None
+    extra = {"sweep": {"unfused_algorithmic_bytes": unfused, "adjoint_ms_per_step": round(adj_ms, 3),
+                       "effective_GBps": round(unfused / (adj_ms * 1e-3) / 1e9, 1),
+                       "effective_frac_of_peak": round(unfused / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
+                       "ops_per_adjoint_pass": round(len(ops[0]) / passes, 1) if passes else None,
+                       "params_per_s_at_70pct_unfused": round(
+                           ntp / (unfused / (0.7 * HBM_PEAK_GBS * 1e9) + (ms - adj_ms) * 1e-3), 1)}}
     if args.c64_steps > 0:
         log("bench: complex64 (FP64 reductions)")
         jac64, dt64, _, _ = run_gpu(capi, n, ops, obs, np.complex64, args.c64_steps, 1, profile=False)
