@@ -271,6 +271,7 @@ void prepare(PlanOp &p) {
 }
 
 constexpr size_t kPassMaxOps = 512;
+constexpr size_t kBundleLookahead = 64;
 constexpr size_t kLookahead = 512;
 
 // Splits a pass's ops (in order) into bundles of <= 2 tile bits.
@@ -279,43 +280,58 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
     for (size_t q = 0; q < ps.tb.size(); q++) {
         pos[ps.tb[q]] = static_cast<int>(q);
     }
-    Bundle cur;
-    uint32_t cb = 0; // tile positions of the current bundle
-    auto flush = [&]() {
-        if (!cur.ops.empty()) {
-            const int lo = __builtin_ctz(cb);
-            if (__builtin_popcount(cb) == 1) {
-                cur.K = 1;
-                cur.qa = lo;
-            } else {
-                cur.K = 2;
-                cur.qa = 31 - __builtin_clz(cb);
-                cur.qb = lo;
-            }
-            ps.bundles.push_back(std::move(cur));
-        }
-        cur = Bundle{};
-        cb = 0;
-    };
-    for (auto &p : ops) {
+    auto tile_mask = [&](const PlanOp &p) {
         uint32_t ob = 0;
         for (auto w : p.U.wires) {
             ob |= 1u << pos[n - 1 - static_cast<int>(w)];
         }
-        if (__builtin_popcount(cb | ob) > 2 || cur.ops.size() >= kSegMaxOps ||
-            (p.trainable && cur.nslots >= SEG_PMAX)) {
-            flush();
+        return ob;
+    };
+    // Each bundle starts at the first remaining op and takes later ops that
+    // fit its two tile bits, provided no skipped op before them shares a
+    // wire (ops on disjoint wires commute), within a window of kBundleLookahead.
+    std::vector<PlanOp> rem = std::move(ops);
+    while (!rem.empty()) {
+        Bundle cur;
+        uint32_t cb = 0;
+        uint64_t blocked = 0;
+        std::vector<PlanOp> rest;
+        for (size_t k = 0; k < rem.size(); k++) {
+            PlanOp &p = rem[k];
+            uint64_t wmask = 0;
+            for (auto w : p.U.wires) {
+                wmask |= uint64_t{1} << w;
+            }
+            const uint32_t ob = tile_mask(p);
+            const bool ok = k < kBundleLookahead && !(blocked & wmask) &&
+                            __builtin_popcount(cb | ob) <= 2 && cur.ops.size() < kSegMaxOps &&
+                            (!p.trainable || cur.nslots < SEG_PMAX);
+            if (ok) {
+                cb |= ob;
+                if (p.trainable) {
+                    cur.nslots++;
+                    ps.cols.push_back(p.col);
+                    ps.coefs.push_back(p.coef);
+                    ps.nslots++;
+                }
+                cur.ops.push_back(std::move(p));
+            } else {
+                blocked |= wmask;
+                rest.push_back(std::move(p));
+            }
         }
-        cb |= ob;
-        if (p.trainable) {
-            cur.nslots++;
-            ps.cols.push_back(p.col);
-            ps.coefs.push_back(p.coef);
-            ps.nslots++;
+        const int lo = __builtin_ctz(cb);
+        if (__builtin_popcount(cb) == 1) {
+            cur.K = 1;
+            cur.qa = lo;
+        } else {
+            cur.K = 2;
+            cur.qa = 31 - __builtin_clz(cb);
+            cur.qb = lo;
         }
-        cur.ops.push_back(std::move(p));
+        ps.bundles.push_back(std::move(cur));
+        rem = std::move(rest);
     }
-    flush();
 }
 
 // Groups operations into LDS tile passes of T bits that always contain the
