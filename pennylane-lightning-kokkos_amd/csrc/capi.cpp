@@ -435,7 +435,7 @@ int tile_bits_forward(int n, int dtype) {
 // Largest observable chunk whose tiles fit the LDS budget.
 int tile_max_nj(int dtype, int T) {
     int best = 1;
-    const int cap = env_int("PLK_TILE_NJ", dtype == PLK_C64 ? 6 : 4);
+    const int cap = env_int("PLK_TILE_NJ", 4);
     for (int nj = 1; nj <= 8; nj++) {
         if (tile_nj_supported(dtype, nj) &&
             tile_lds_bytes(dtype, T, nj, TILE_PMAX) <= TILE_LDS_MAX && nj <= cap) {
@@ -1345,8 +1345,7 @@ void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double 
     Device &dev = device(device_id);
 
     // Longest-processing-time-first packing of circuits onto workgroups.
-    const size_t lds = batch_lds_bytes(dtype, tile_bits) + sizeof(double) * 1024;
-    const int per_cu = std::max(1, std::min(8, static_cast<int>((160 * 1024) / lds)));
+    const int per_cu = batch_resident_blocks(dtype, tile_bits);
     const int grid = static_cast<int>(
         std::min<size_t>(in.size(), static_cast<size_t>(dev.num_cus) * per_cu));
     std::vector<int32_t> by_cost(in.size());

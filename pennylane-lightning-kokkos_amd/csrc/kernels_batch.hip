@@ -284,6 +284,16 @@ __global__ void __launch_bounds__(256) zero_kernel(double *p, int64_t n) {
     }
 }
 
+template <typename T> void set_batch_lds_limit() {
+    static bool attr = false;
+    if (!attr) {
+        PLK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<T>),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    static_cast<int>(batch_lds_bytes(PLK_C128, BATCH_TMAX))));
+        attr = true;
+    }
+}
+
 template <typename T> void launch_t(Device &dev, const BatchArgs &A) {
     if (A.jac_len > 0) {
         const unsigned g = static_cast<unsigned>(std::min<int64_t>((A.jac_len + 255) / 256, 1024));
@@ -294,13 +304,7 @@ template <typename T> void launch_t(Device &dev, const BatchArgs &A) {
         return;
     }
     const size_t lds = batch_lds_bytes(sizeof(T) == 4 ? PLK_C64 : PLK_C128, A.tile_bits);
-    static bool attr = false;
-    if (!attr) {
-        PLK_HIP(hipFuncSetAttribute(reinterpret_cast<const void *>(&batch_kernel<T>),
-                                    hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    static_cast<int>(batch_lds_bytes(PLK_C128, BATCH_TMAX))));
-        attr = true;
-    }
+    set_batch_lds_limit<T>();
     ProfScope prof("adjoint_batch", 0.0, dev.stream);
     hipLaunchKernelGGL(batch_kernel<T>, dim3(A.grid), dim3(BATCH_BLOCK), lds, dev.stream, A);
     PLK_HIP(hipGetLastError());
@@ -310,6 +314,21 @@ template <typename T> void launch_t(Device &dev, const BatchArgs &A) {
 
 size_t batch_lds_bytes(int dtype, int tile_bits) {
     return 2 * (size_t{1} << tile_bits) * (dtype == PLK_C64 ? 8 : 16);
+}
+
+int batch_resident_blocks(int dtype, int tile_bits) {
+    const size_t lds = batch_lds_bytes(dtype, tile_bits);
+    int nb = 0;
+    if (dtype == PLK_C64) {
+        set_batch_lds_limit<float>();
+        PLK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&batch_kernel<float>), BATCH_BLOCK, lds));
+    } else {
+        set_batch_lds_limit<double>();
+        PLK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+            &nb, reinterpret_cast<const void *>(&batch_kernel<double>), BATCH_BLOCK, lds));
+    }
+    return std::max(1, std::min(nb, 8));
 }
 
 void launch_batch(Device &dev, int dtype, const BatchArgs &A) {

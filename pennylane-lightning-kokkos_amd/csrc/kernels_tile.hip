@@ -13,6 +13,9 @@
 // tiles, so the reduction is deterministic. One launch reads and writes each
 // state once: (1 + w + 2 nj) S bytes for however many operations the pass
 // holds.
+#include <cstdlib>
+#include <map>
+
 #include "device_common.hpp"
 
 namespace plk {
@@ -225,6 +228,28 @@ void set_lds_limit() {
     }
 }
 
+// Blocks of `kernel` that fit on one CU with `lds` bytes of dynamic LDS
+// (registers and LDS both counted by the runtime); PLK_TILE_BLOCKS_PER_CU
+// overrides. A grid of exactly the resident blocks avoids a second, partial
+// round of blocks in the grid-stride loop.
+int resident_blocks(const void *kernel, size_t lds) {
+    const char *e = std::getenv("PLK_TILE_BLOCKS_PER_CU");
+    if (e && *e) {
+        return std::max(1, std::min(std::atoi(e), 8));
+    }
+    static std::map<std::pair<const void *, size_t>, int> cache;
+    const auto key = std::make_pair(kernel, lds);
+    auto it = cache.find(key);
+    if (it != cache.end()) {
+        return it->second;
+    }
+    int nb = 0;
+    PLK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kernel, TILE_BLOCK, lds));
+    nb = std::max(1, std::min(nb, 8));
+    cache[key] = nb;
+    return nb;
+}
+
 template <typename T, int NJ>
 int tile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void *d_bundles,
                   const void *d_ops, const TileLaunch &L) {
@@ -237,7 +262,15 @@ int tile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void
         raise(PLK_ERR_INVALID, "tile pass exceeds the LDS budget");
     }
     const uint64_t ntiles = uint64_t{1} << (L.n - L.T);
-    const uint64_t cap = static_cast<uint64_t>(dev.num_cus) * TILE_BLOCKS_PER_CU;
+    const bool small = phi.bytes() <= (uint64_t{1} << 32);
+    const void *kfn = small ? reinterpret_cast<const void *>(&tile_kernel<T, NJ, uint32_t>)
+                            : reinterpret_cast<const void *>(&tile_kernel<T, NJ, uint64_t>);
+    if (small) {
+        set_lds_limit<T, NJ, uint32_t>();
+    } else {
+        set_lds_limit<T, NJ, uint64_t>();
+    }
+    const uint64_t cap = static_cast<uint64_t>(dev.num_cus) * resident_blocks(kfn, lds);
     const unsigned grid = static_cast<unsigned>(std::min(ntiles, cap));
     if (NJ > 0 && L.nslots > 0) {
         ensure_red(dev, static_cast<size_t>(grid) * L.nslots * NJ, 0);

@@ -197,7 +197,6 @@ void launch_segment_reduce(Device &dev, int grid, int nslots, int nj, int j0,
 constexpr int TILE_BLOCK = 256;
 constexpr int TILE_TMAX = 14;          // tile bits
 constexpr int TILE_PMAX = 24;          // trainable ops per pass
-constexpr int TILE_BLOCKS_PER_CU = 2;
 constexpr int TILE_ROWS = TILE_BLOCK / 16; // 16-lane rows holding FP64 partials
 constexpr size_t TILE_LDS_MAX = 80 * 1024;
 
@@ -283,6 +282,8 @@ struct BatchArgs {
 };
 
 size_t batch_lds_bytes(int dtype, int tile_bits);
+// Workgroups of the batch kernel resident per CU at this tile width.
+int batch_resident_blocks(int dtype, int tile_bits);
 void launch_batch(Device &dev, int dtype, const BatchArgs &A);
 
 void launch_dense(Device &dev, const State &s, const CompiledOp &op);
