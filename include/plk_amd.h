@@ -220,6 +220,29 @@ int plk_adjoint_batched(int dtype, int device_id, size_t num_circuits,
                         const size_t *num_trainable, double *expvals,
                         double *jac);
 
+/* ---- probabilities and samples: LightningKokkos_C*.probs / GenerateSamples
+ *      (Bindings.cpp:517-553; MeasuresKokkos.hpp:389-569) --------------- */
+
+/* MeasuresKokkos::probs(wires) (MeasuresKokkos.hpp:413-520): the 2^num_wires
+ * marginal probabilities of `wires`, FP64 sums, deterministic. For sorted
+ * wires, output index bits follow the wires (wires[0] most significant). For
+ * unsorted wires the reference's transposition is reproduced exactly: the
+ * output is the marginal in the order wires[pi[pi[k]]], pi = argsort(wires)
+ * (equal to the given order whenever that order is an involution).
+ * num_wires == 0 means every wire in order (MeasuresKokkos::probs(),
+ * 389-411). */
+int plk_probs(const plk_state *sv, const size_t *wires, size_t num_wires,
+              double *probs);
+/* MeasuresKokkos::generate_samples (MeasuresKokkos.hpp:522-569): num_shots
+ * samples of the full register by inverse transform of the FP64 exclusive
+ * CDF. samples[s * n + (n - 1 - j)] = bit j of shot s's basis index (a row
+ * per shot, wire 0 first), as the reference lays them out. Uniforms are
+ * output s of SplitMix64 seeded with `seed` (the reference draws from
+ * Kokkos' XorShift64 pool, so individual samples differ; distributions
+ * agree). */
+int plk_generate_samples(const plk_state *sv, size_t num_shots, uint64_t seed,
+                         size_t *samples);
+
 /* ---- measurement support ---------------------------------------------- */
 
 /* Enables (1) or disables (0) per-kernel timing with HIP events on the

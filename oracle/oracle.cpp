@@ -27,7 +27,11 @@
 //     and tests/test_hamiltonian*.py, plus dense numpy brute force;
 //   * adjoint Jacobian: goldens of src/tests/Test_AdjointDiffKokkos.cpp, central
 //     finite differences and exact parameter shift on random circuits;
-//   * CSR SpMV: goldens of src/tests/Test_LinearAlgebra.cpp.
+//   * CSR SpMV: goldens of src/tests/Test_LinearAlgebra.cpp;
+//   * probabilities: goldens of src/tests/Test_StateVectorKokkos_Measure.cpp
+//     and tests/test_probs.py, plus numpy marginalisation of |psi|^2;
+//   * sampling (index selection for given uniforms): the bracketing property
+//     cdf[k] < u <= cdf[k+1] and basis states.
 
 #include <algorithm>
 #include <cmath>
@@ -800,6 +804,99 @@ static Mat to_mat(const double *m, size_t len) {
     return M;
 }
 
+// ---------------------------------------------------------------------------
+// Probabilities and sampling (simulator/MeasuresKokkos.hpp:389-569,
+// simulator/MeasuresFunctors.hpp:15-210, util/UtilKokkos.hpp:353-581).
+// ---------------------------------------------------------------------------
+
+// generateBitsPatterns (UtilKokkos.hpp:567-583): the amplitude offsets of the
+// 2^k assignments of `qubits`, the first qubit most significant.
+std::vector<size_t> bit_patterns(const std::vector<size_t> &qubits, int n) {
+    std::vector<size_t> idx{0};
+    for (size_t it0 = 0; it0 < qubits.size(); it0++) {
+        const size_t q = qubits[qubits.size() - 1 - it0];
+        const size_t value = size_t{1} << (n - 1 - static_cast<int>(q));
+        const size_t cur = idx.size();
+        for (size_t j = 0; j < cur; j++) {
+            idx.push_back(idx[j] + value);
+        }
+    }
+    return idx;
+}
+
+// MeasuresKokkos::probs(wires) (MeasuresKokkos.hpp:413-520): sub-probabilities
+// over the sorted wires, then the transposition to the given wire order
+// (getTransposedIndexFunctor, MeasuresFunctors.hpp:144-175).
+std::vector<double> probs_of(const cd *sv, int n, const std::vector<size_t> &wires) {
+    std::vector<size_t> order(wires.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        order[k] = k;
+    }
+    std::sort(order.begin(), order.end(),
+              [&](size_t a, size_t b) { return wires[a] < wires[b]; });
+    std::vector<size_t> sorted(wires.size());
+    for (size_t k = 0; k < order.size(); k++) {
+        sorted[k] = wires[order[k]];
+    }
+    std::vector<size_t> rest;
+    for (int q = 0; q < n; q++) {
+        if (std::find(sorted.begin(), sorted.end(), static_cast<size_t>(q)) == sorted.end()) {
+            rest.push_back(static_cast<size_t>(q));
+        }
+    }
+    const auto idx = bit_patterns(sorted, n);
+    const auto off = bit_patterns(rest, n);
+    std::vector<double> sub(idx.size(), 0.0);
+    for (size_t i = 0; i < idx.size(); i++) {
+        for (size_t j = 0; j < off.size(); j++) {
+            sub[i] += std::norm(sv[idx[i] + off[j]]);
+        }
+    }
+    const size_t m = wires.size();
+    std::vector<double> out(idx.size());
+    for (size_t i = 0; i < idx.size(); i++) {
+        size_t t = 0;
+        for (size_t j = 0; j < m; j++) {
+            const size_t bit = (i >> (m - 1 - j)) & 1;
+            t += bit << (m - 1 - order[j]);
+        }
+        out[i] = sub[t];
+    }
+    return out;
+}
+
+// generate_samples (MeasuresKokkos.hpp:533-569) with the uniforms given:
+// exclusive CDF (getCDFFunctor) and the binary search of Sampler
+// (MeasuresFunctors.hpp:74-110). Returns basis indices.
+void sample_indices(const cd *sv, int n, const double *u, size_t shots, int64_t *out) {
+    const size_t N = size_t{1} << n;
+    std::vector<double> cdf(N);
+    double run = 0.0;
+    for (size_t k = 0; k < N; k++) {
+        cdf[k] = run;
+        run += std::norm(sv[k]);
+    }
+    for (size_t s = 0; s < shots; s++) {
+        size_t index;
+        if (u[s] <= cdf[1]) {
+            index = 0;
+        } else {
+            size_t low = 1, high = N;
+            while (high - low > 1) {
+                const size_t mid = high - ((high - low) >> 1);
+                const double c = mid == N ? 1.0 : cdf[mid];
+                if (c < u[s]) {
+                    low = mid;
+                } else {
+                    high = mid;
+                }
+            }
+            index = high - 1;
+        }
+        out[s] = static_cast<int64_t>(index);
+    }
+}
+
 extern "C" {
 
 const char *orc_last_error(void) { return g_last_error.c_str(); }
@@ -1081,6 +1178,30 @@ int orc_adjoint_jacobian(const double *sv, int n, void *const *obs, int n_obs,
         }
         adjoint_jacobian(reinterpret_cast<const cd *>(sv), n, ob,
                          *static_cast<const Ops *>(ops), t, apply_ops != 0, jac);
+    })
+}
+
+int orc_probs(const double *sv, int n, const int64_t *wires, int nw, double *out) {
+    ORC_TRY({
+        std::vector<size_t> w;
+        for (int k = 0; k < nw; k++) {
+            w.push_back(static_cast<size_t>(wires[k]));
+        }
+        if (w.empty()) {
+            for (int q = 0; q < n; q++) {
+                w.push_back(static_cast<size_t>(q));
+            }
+        }
+        const auto p = probs_of(reinterpret_cast<const cd *>(sv), n, w);
+        std::copy(p.begin(), p.end(), out);
+    })
+}
+
+int orc_sample_indices(const double *sv, int n, const double *uniforms, int64_t shots,
+                       int64_t *out) {
+    ORC_TRY({
+        sample_indices(reinterpret_cast<const cd *>(sv), n, uniforms,
+                       static_cast<size_t>(shots), out);
     })
 }
 

@@ -175,6 +175,26 @@ def expval_csr(sv: np.ndarray, data, indices, indptr) -> float:
     return out.value
 
 
+def probs(sv: np.ndarray, wires=()) -> np.ndarray:
+    """MeasuresKokkos::probs(wires): marginals, index bits in the given wire order."""
+    n = int(np.log2(sv.size))
+    w = _as_i64(list(wires))
+    out = np.zeros(2 ** (w.size if w.size else n))
+    x = np.ascontiguousarray(sv, dtype=np.complex128)
+    _check(lib().orc_probs(_dp(x), n, _ip(w), int(w.size), _dp(out)))
+    return out
+
+
+def sample_indices(sv: np.ndarray, uniforms) -> np.ndarray:
+    """Basis indices that generate_samples selects for the given uniforms."""
+    n = int(np.log2(sv.size))
+    u = np.ascontiguousarray(uniforms, dtype=np.float64)
+    out = np.zeros(max(1, u.size), dtype=np.int64)
+    x = np.ascontiguousarray(sv, dtype=np.complex128)
+    _check(lib().orc_sample_indices(_dp(x), n, _dp(u), ctypes.c_int64(u.size), _ip(out)))
+    return out[: u.size]
+
+
 # ----------------------------------------------------------------------------
 # Observables (ObservablesKokkos.hpp:19-517) as plain Python descriptions;
 # converted to oracle handles on demand.

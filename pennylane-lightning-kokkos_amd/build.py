@@ -32,7 +32,8 @@ HIP_FLAGS = [
 ]
 
 LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "kernels_apply.hip",
-               "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip"]
+               "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip",
+               "kernels_sample.hip"]
 
 
 def _run(cmd, verbose):

@@ -317,6 +317,32 @@ template <class PrecisionT> void bind_precision(py::module_ &m) {
                  return out;
              },
              "Calculate the expectation value of the given observable.")
+        .def("probs",
+             [](SV &sv, const std::vector<size_t> &wires) {
+                 const size_t m = wires.empty() ? static_cast<size_t>(sv.n) : wires.size();
+                 std::vector<double> p(size_t{1} << m);
+                 chk(plk_probs(sv.h, wires.data(), wires.size(), p.data()));
+                 py::array_t<PrecisionT> out(static_cast<py::ssize_t>(p.size()));
+                 auto *o = out.mutable_data();
+                 for (size_t i = 0; i < p.size(); i++) {
+                     o[i] = static_cast<PrecisionT>(p[i]);
+                 }
+                 return out;
+             })
+        .def("GenerateSamples",
+             [](SV &sv, size_t num_wires, size_t num_shots) {
+                 // The reference draws from a pool seeded with 5374857
+                 // (MeasuresKokkos.hpp:551); the same seed feeds SplitMix64 here.
+                 py::array_t<size_t> out({static_cast<py::ssize_t>(num_shots),
+                                          static_cast<py::ssize_t>(num_wires)});
+                 if (num_wires != static_cast<size_t>(sv.n)) {
+                     throw EngineError("GenerateSamples: num_wires must equal the qubit count");
+                 }
+                 if (num_shots > 0) {
+                     chk(plk_generate_samples(sv.h, num_shots, 5374857, out.mutable_data()));
+                 }
+                 return out;
+             })
         .def("DeviceToHost",
              [](SV &sv, np_c &host) {
                  auto b = host.request(true);

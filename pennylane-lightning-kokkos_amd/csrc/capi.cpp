@@ -1963,6 +1963,59 @@ int plk_adjoint_batched(int dtype, int device_id, size_t num_circuits, const int
     })
 }
 
+int plk_probs(const plk_state *sv, const size_t *wires, size_t num_wires, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        if (!out || (num_wires > 0 && !wires)) {
+            raise(PLK_ERR_INVALID, "null argument");
+        }
+        std::vector<size_t> w(wires, wires + num_wires);
+        if (w.empty()) {
+            for (int q = 0; q < s.n; q++) {
+                w.push_back(static_cast<size_t>(q));
+            }
+        }
+        uint64_t seen = 0;
+        for (size_t q : w) {
+            if (q >= static_cast<size_t>(s.n)) {
+                raise(PLK_ERR_INVALID, "wire " + std::to_string(q) + " out of range");
+            }
+            if (seen >> q & 1) {
+                raise(PLK_ERR_INVALID, "repeated wire in probs");
+            }
+            seen |= uint64_t{1} << q;
+        }
+        // The reference sums over the sorted wires and then transposes with
+        // getTransposedIndexFunctor (MeasuresKokkos.hpp:487-512,
+        // MeasuresFunctors.hpp:144-175); its output equals the marginal whose
+        // index bits follow the wire order w[pi[pi[k]]], pi = argsort(w).
+        std::vector<size_t> pi(w.size());
+        for (size_t k = 0; k < pi.size(); k++) {
+            pi[k] = k;
+        }
+        std::sort(pi.begin(), pi.end(), [&](size_t a, size_t b) { return w[a] < w[b]; });
+        std::vector<int> g;
+        for (size_t k = 0; k < w.size(); k++) {
+            g.push_back(s.n - 1 - static_cast<int>(w[pi[pi[k]]]));
+        }
+        Device &dev = dev_of(s);
+        set_device(s.device);
+        probs(dev, s, g, out);
+    })
+}
+
+int plk_generate_samples(const plk_state *sv, size_t num_shots, uint64_t seed, size_t *samples) {
+    PLK_TRY({
+        const State &s = S(sv);
+        if (num_shots > 0 && !samples) {
+            raise(PLK_ERR_INVALID, "null samples buffer");
+        }
+        Device &dev = dev_of(s);
+        set_device(s.device);
+        sample(dev, s, num_shots, seed, samples);
+    })
+}
+
 int plk_profiling_enable(int on) { PLK_TRY({ profiling_enable_impl(on != 0); }) }
 
 int plk_profiling_get(const char *kernel, double *launches, double *total_ms,

@@ -323,6 +323,12 @@ void csr_spmv(Device &dev, const State &out, const State &in, const double *data
               const int64_t *indices, const int64_t *indptr, size_t nnz,
               size_t rows);
 
+// Probabilities and sampling (kernels_sample.hip). probs: 2^m FP64 values,
+// bin bit (m-1-k) taken from global bit gbits[k]. sample: shots rows of n
+// bits, wire 0 first.
+void probs(Device &dev, const State &s, const std::vector<int> &gbits, double *host_out);
+void sample(Device &dev, const State &s, size_t shots, uint64_t seed, size_t *host_out);
+
 } // namespace plk
 
 // Opaque handle types of the C ABI.

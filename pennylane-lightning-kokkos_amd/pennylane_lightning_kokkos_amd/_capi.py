@@ -76,6 +76,8 @@ SIGNATURES = {
                                            ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_vp),
                                            ctypes.POINTER(_vp), ctypes.POINTER(_c_size_p),
                                            _c_size_p, _c_double_p, _c_double_p]),
+    "plk_probs": (ctypes.c_int, [_vp, _c_size_p, ctypes.c_size_t, _c_double_p]),
+    "plk_generate_samples": (ctypes.c_int, [_vp, ctypes.c_size_t, ctypes.c_uint64, _c_size_p]),
     "plk_profiling_enable": (ctypes.c_int, [ctypes.c_int]),
     "plk_profiling_get": (ctypes.c_int, [ctypes.c_char_p, _c_double_p, _c_double_p, _c_double_p]),
     "plk_profiling_names": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_size_t]),
@@ -187,6 +189,21 @@ class State:
 
     def synchronize(self):
         check(load().plk_state_synchronize(self.h))
+
+    def probs(self, wires: Sequence[int] = ()) -> np.ndarray:
+        """Marginal probabilities of `wires` (all wires when empty), in the
+        reference's order (MeasuresKokkos::probs; see plk_probs)."""
+        w = _sizes(wires)
+        out = np.zeros(2 ** (w.size if w.size else self.num_qubits), dtype=np.float64)
+        check(load().plk_probs(self.h, _sp(w), w.size, _dp(out)))
+        return out
+
+    def generate_samples(self, shots: int, seed: int = 5374857) -> np.ndarray:
+        """shots x num_qubits array of 0/1 (wire 0 first)."""
+        out = np.zeros(max(1, shots * self.num_qubits), dtype=np.uint64)
+        check(load().plk_generate_samples(self.h, int(shots), int(seed) & (2**64 - 1),
+                                          out.ctypes.data_as(_c_size_p)))
+        return out[: shots * self.num_qubits].reshape(shots, self.num_qubits)
 
     def device_ptr(self) -> int:
         p = _vp()

@@ -221,3 +221,36 @@ def c4_workload(count=4096, seed=20261205, nmin=6, nmax=16, dmin=2, dmax=8):
         obs = ("ham", [1.0] * n, [("named", "PauliZ", [w]) for w in range(n)])
         out.append((n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs))
     return out
+
+
+# ----------------------------------------------------------------------------
+# Sampling helpers.
+# ----------------------------------------------------------------------------
+SPLITMIX_GAMMA = 0x9E3779B97F4A7C15
+MASK64 = (1 << 64) - 1
+
+
+def splitmix64(k_plus_1: int, seed: int) -> int:
+    """Output k of the SplitMix64 generator seeded with `seed` (Steele, Lea
+    and Flood 2014): mix(seed + (k + 1) * gamma)."""
+    z = (seed + k_plus_1 * SPLITMIX_GAMMA) & MASK64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & MASK64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & MASK64
+    return z ^ (z >> 31)
+
+
+def sample_uniforms(seed: int, shots: int) -> np.ndarray:
+    """The uniforms in (0, 1] that plk_generate_samples draws for each shot."""
+    return np.array([((splitmix64(s + 1, seed) >> 11) + 1) * 2.0 ** -53 for s in range(shots)])
+
+
+def initializing_state(n=3) -> np.ndarray:
+    """Initializing_StateVector of the reference tests (TestHelpers.hpp:154-177):
+    RX(a), RY(a) on each wire with a = 0.7, 0.5, 0.3, ..."""
+    sv = O.zero_state(n)
+    a = 0.7
+    for w in range(n):
+        O.apply(sv, "RX", [w], False, [a])
+        O.apply(sv, "RY", [w], False, [a])
+        a -= 0.2
+    return sv
