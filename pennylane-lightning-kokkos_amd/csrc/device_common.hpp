@@ -398,7 +398,10 @@ __device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1
 // the constant address space so the uniform loads become scalar loads) to
 // the register group f of phi and the groups h[j] of NJ observable states.
 // For a trainable op the FP64 inner products Im<h_j | G f> are added to
-// acc[slot][j] before the op is applied.
+// acc[slot][j] before the op is applied. The trainable ops of a group hold
+// slots 0, 1, 2, ... in op order, so their products enter a register queue
+// (newest first) and reach acc[slot] once per group, after the loop: the
+// loop body needs no selects over the slots.
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? NJ : 1][1 << K],
                                         double (&acc)[SEG_PMAX][NJ > 0 ? NJ : 1],
@@ -407,6 +410,16 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
     using CT = __attribute__((address_space(4))) const T;
     const auto *base = (const __attribute__((address_space(4))) unsigned char *)(group);
     const auto *heads = reinterpret_cast<const __attribute__((address_space(4))) OpHead *>(base);
+    constexpr int NA = NJ > 0 ? NJ : 1;
+    double q[SEG_PMAX][NA];
+#pragma unroll
+    for (int s = 0; s < SEG_PMAX; s++) {
+#pragma unroll
+        for (int j = 0; j < NA; j++) {
+            q[s][j] = 0.0;
+        }
+    }
+    int nt = 0;
     for (int o = 0; o < nops; o++) {
         const int cls = heads[o].cls;
         const int slot = heads[o].slot;
@@ -429,12 +442,12 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
                     for (int j = 0; j < NJ; j++) {
                         const double d = im_dot(h[j], gf);
 #pragma unroll
-                        for (int s = 0; s < SEG_PMAX; s++) {
-                            if (s == slot) {
-                                acc[s][j] += d;
-                            }
+                        for (int s = SEG_PMAX - 1; s > 0; s--) {
+                            q[s][j] = q[s - 1][j];
                         }
+                        q[0][j] = d;
                     }
+                    nt++;
                 }
             }
             apply_kind<T, K, m, qa, qb>(f, ukind, swp, U);
@@ -445,6 +458,29 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
                 }
             }
         });
+    }
+    if constexpr (NJ > 0) {
+        // Align the queue so that q[SEG_PMAX - 1 - s] holds slot s.
+#pragma unroll
+        for (int r = 0; r < SEG_PMAX; r++) {
+            if (r >= nt) {
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+#pragma unroll
+                    for (int s = SEG_PMAX - 1; s > 0; s--) {
+                        q[s][j] = q[s - 1][j];
+                    }
+                    q[0][j] = 0.0;
+                }
+            }
+        }
+#pragma unroll
+        for (int s = 0; s < SEG_PMAX; s++) {
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                acc[s][j] += q[SEG_PMAX - 1 - s][j];
+            }
+        }
     }
 }
 
