@@ -428,7 +428,7 @@ int tile_bits_forward(int n, int dtype) {
     if (env_int("PLK_TILES", 1) == 0) {
         return 0;
     }
-    const int T = std::min(env_int("PLK_TILE_BITS_FWD", dtype == PLK_C64 ? 13 : 12), n - 8);
+    const int T = std::min(env_int("PLK_TILE_BITS_FWD", dtype == PLK_C64 ? 11 : 10), n - 8);
     return T >= low_bits(dtype) + 2 ? T : 0;
 }
 
