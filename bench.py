@@ -46,7 +46,10 @@ def measured_traffic(family):
     rec = json.load(open(TRAFFIC_JSON))
     if rec.get("family") != family:
         return None
-    src = sorted(glob.glob(os.path.join(REPO, "pennylane-lightning-kokkos_amd", "csrc", "*")))
+    # Device code only: the bytes a launch moves do not depend on host planning.
+    csrc = os.path.join(REPO, "pennylane-lightning-kokkos_amd", "csrc")
+    src = sorted(glob.glob(os.path.join(csrc, "*.hip")) +
+                 [os.path.join(csrc, "device_common.hpp"), os.path.join(csrc, "plk_internal.hpp")])
     h = hashlib.sha256(b"".join(open(p, "rb").read() for p in src)).hexdigest()[:16]
     return rec["traffic_bytes_per_launch"] if rec.get("source_hash") == h else None
 

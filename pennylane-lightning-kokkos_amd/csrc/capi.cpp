@@ -280,34 +280,38 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
     for (size_t q = 0; q < ps.tb.size(); q++) {
         pos[ps.tb[q]] = static_cast<int>(q);
     }
-    auto tile_mask = [&](const PlanOp &p) {
+    std::vector<PlanOp> all = std::move(ops);
+    std::vector<uint32_t> obits(all.size());
+    std::vector<uint64_t> wmask(all.size());
+    std::vector<int> rem(all.size());
+    for (size_t i = 0; i < all.size(); i++) {
         uint32_t ob = 0;
-        for (auto w : p.U.wires) {
+        uint64_t wm = 0;
+        for (auto w : all[i].U.wires) {
             ob |= 1u << pos[n - 1 - static_cast<int>(w)];
+            wm |= uint64_t{1} << w;
         }
-        return ob;
-    };
+        obits[i] = ob;
+        wmask[i] = wm;
+        rem[i] = static_cast<int>(i);
+    }
     // Each bundle starts at the first remaining op and takes later ops that
     // fit its two tile bits, provided no skipped op before them shares a
     // wire (ops on disjoint wires commute), within a window of kBundleLookahead.
-    std::vector<PlanOp> rem = std::move(ops);
+    std::vector<int> rest;
     while (!rem.empty()) {
         Bundle cur;
         uint32_t cb = 0;
         uint64_t blocked = 0;
-        std::vector<PlanOp> rest;
-        for (size_t k = 0; k < rem.size(); k++) {
-            PlanOp &p = rem[k];
-            uint64_t wmask = 0;
-            for (auto w : p.U.wires) {
-                wmask |= uint64_t{1} << w;
-            }
-            const uint32_t ob = tile_mask(p);
-            const bool ok = k < kBundleLookahead && !(blocked & wmask) &&
-                            __builtin_popcount(cb | ob) <= 2 && cur.ops.size() < kSegMaxOps &&
-                            (!p.trainable || cur.nslots < SEG_PMAX);
+        rest.clear();
+        size_t k = 0;
+        for (; k < rem.size() && k < kBundleLookahead; k++) {
+            const int i = rem[k];
+            PlanOp &p = all[i];
+            const bool ok = !(blocked & wmask[i]) && __builtin_popcount(cb | obits[i]) <= 2 &&
+                            cur.ops.size() < kSegMaxOps && (!p.trainable || cur.nslots < SEG_PMAX);
             if (ok) {
-                cb |= ob;
+                cb |= obits[i];
                 if (p.trainable) {
                     cur.nslots++;
                     ps.cols.push_back(p.col);
@@ -316,10 +320,11 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
                 }
                 cur.ops.push_back(std::move(p));
             } else {
-                blocked |= wmask;
-                rest.push_back(std::move(p));
+                blocked |= wmask[i];
+                rest.push_back(i);
             }
         }
+        rest.insert(rest.end(), rem.begin() + static_cast<long>(k), rem.end());
         const int lo = __builtin_ctz(cb);
         if (__builtin_popcount(cb) == 1) {
             cur.K = 1;
@@ -330,7 +335,7 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
             cur.qb = lo;
         }
         ps.bundles.push_back(std::move(cur));
-        rem = std::move(rest);
+        rem.swap(rest);
     }
 }
 
@@ -340,55 +345,61 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
 // is a barrier and runs on its own.
 std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
                              int pmax = TILE_PMAX) {
-    std::vector<PlanOp> rem;
+    std::vector<PlanOp> all;
     for (auto &p : ops_in) {
         if (p.U.kind == OpKind::Identity && !p.trainable) {
             continue;
         }
         prepare(p);
-        rem.push_back(std::move(p));
+        all.push_back(std::move(p));
+    }
+    // Wire and state-bit masks (wires and bits < 64).
+    std::vector<uint64_t> wmask(all.size()), bmask(all.size());
+    std::vector<char> fus(all.size());
+    std::vector<int> rem(all.size());
+    for (size_t i = 0; i < all.size(); i++) {
+        for (auto w : all[i].U.wires) {
+            wmask[i] |= uint64_t{1} << w;
+            bmask[i] |= uint64_t{1} << (n - 1 - static_cast<int>(w));
+        }
+        fus[i] = fusable(all[i]) ? 1 : 0;
+        rem[i] = static_cast<int>(i);
     }
     std::vector<Step> steps;
-    size_t head = 0;
-    while (head < rem.size()) {
-        if (!fusable(rem[head])) {
+    std::vector<int> taken, rest;
+    while (!rem.empty()) {
+        if (!fus[rem[0]]) {
             Step s;
-            s.single = std::move(rem[head]);
+            s.single = std::move(all[rem[0]]);
             steps.push_back(std::move(s));
-            head++;
+            rem.erase(rem.begin());
             continue;
         }
-        // Tile bits and blocked wires as masks (state bits and wires < 64).
         uint64_t bits = L > 0 ? (uint64_t{1} << L) - 1 : 0;
         uint64_t blocked = 0;
-        std::vector<PlanOp> taken;
-        std::vector<PlanOp> rest;
+        taken.clear();
+        rest.clear();
         int nslots = 0;
-        size_t k = head;
+        size_t k = 0;
         for (; k < rem.size(); k++) {
-            PlanOp &p = rem[k];
-            if (!fusable(p) || k - head >= kLookahead) {
+            const int i = rem[k];
+            if (!fus[i] || k >= kLookahead) {
                 break;
             }
-            uint64_t wmask = 0, nb = bits;
-            for (auto w : p.U.wires) {
-                wmask |= uint64_t{1} << w;
-                nb |= uint64_t{1} << (n - 1 - static_cast<int>(w));
-            }
-            const bool ok = taken.size() < kPassMaxOps && (!p.trainable || nslots < pmax) &&
-                            !(blocked & wmask) && __builtin_popcountll(nb) <= T;
+            const bool tr = all[i].trainable;
+            const uint64_t nb = bits | bmask[i];
+            const bool ok = taken.size() < kPassMaxOps && (!tr || nslots < pmax) &&
+                            !(blocked & wmask[i]) && __builtin_popcountll(nb) <= T;
             if (ok) {
                 bits = nb;
-                nslots += p.trainable ? 1 : 0;
-                taken.push_back(std::move(p));
+                nslots += tr ? 1 : 0;
+                taken.push_back(i);
             } else {
-                blocked |= wmask;
-                rest.push_back(std::move(p));
+                blocked |= wmask[i];
+                rest.push_back(i);
             }
         }
-        for (; k < rem.size(); k++) {
-            rest.push_back(std::move(rem[k]));
-        }
+        rest.insert(rest.end(), rem.begin() + static_cast<long>(k), rem.end());
         // Pad the tile with the lowest unused bits.
         for (int b = 0; __builtin_popcountll(bits) < T && b < n; b++) {
             bits |= uint64_t{1} << b;
@@ -400,10 +411,14 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
                 s.pass.tb.push_back(b);
             }
         }
-        make_bundles(s.pass, std::move(taken), n);
+        std::vector<PlanOp> ops;
+        ops.reserve(taken.size());
+        for (int i : taken) {
+            ops.push_back(std::move(all[i]));
+        }
+        make_bundles(s.pass, std::move(ops), n);
         steps.push_back(std::move(s));
-        rem = std::move(rest);
-        head = 0;
+        rem.swap(rest);
     }
     return steps;
 }
