@@ -137,3 +137,13 @@ def test_batched_trainable_subsets_and_inverse(capi):
         np.testing.assert_allclose(rows[i], jac, rtol=1e-10, atol=1e-12)
         np.testing.assert_allclose(ev[i], e, rtol=1e-10, atol=1e-12)
     assert rows[2][5] == 0.0
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_batched_large_streamed_circuits(capi, dtype):
+    """17-20 qubits at the default tile width: every pass streams 2^(n-T)
+    tiles through LDS; mixed with small resident circuits."""
+    rng = np.random.default_rng(81)
+    circuits, tps = _random_batch(rng, 4, 17, 20, nops=50)
+    small, stps = _random_batch(rng, 6, 4, 9, nops=20)
+    _check(capi, circuits + small, tps + stps, dtype)
