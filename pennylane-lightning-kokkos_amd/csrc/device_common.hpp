@@ -238,54 +238,53 @@ template <int M, int QA, int QB> __device__ __forceinline__ constexpr int op_ind
     return M == 1 ? ((i >> QA) & 1) : ((((i >> QA) & 1) << 1) | ((i >> QB) & 1));
 }
 
-// Exchanges the elements whose op-local indices are A and B (A < B).
+// dst = src with the elements whose op-local indices are A and B exchanged.
 template <typename T, int K, int M, int QA, int QB, int A, int B>
-__device__ __forceinline__ void swap_pair(Cx<T> (&v)[1 << K]) {
+__device__ __forceinline__ void permute_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K]) {
 #pragma unroll
     for (int i = 0; i < (1 << K); i++) {
-        if (op_index<M, QA, QB>(i) != A) {
-            continue;
-        }
+        const int r = op_index<M, QA, QB>(i);
         int j = i;
-        if (M == 1) {
-            j = i ^ (1 << QA);
-        } else {
-            if ((A ^ B) & 2) {
-                j ^= 1 << QA;
-            }
-            if ((A ^ B) & 1) {
-                j ^= 1 << QB;
+        if (r == A || r == B) {
+            if (M == 1) {
+                j = i ^ (1 << QA);
+            } else {
+                if ((A ^ B) & 2) {
+                    j ^= 1 << QA;
+                }
+                if ((A ^ B) & 1) {
+                    j ^= 1 << QB;
+                }
             }
         }
-        const Cx<T> t = v[i];
-        v[i] = v[j];
-        v[j] = t;
+        dst[i] = src[j];
     }
 }
 
-// U applied to one group according to its structure.
+// dst = U src for one group, by the structure of U. src and dst are distinct
+// arrays, so no result aliases an input still to be read.
 template <typename T, int K, int M, int QA, int QB>
-__device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int swap,
-                                           const __attribute__((address_space(4))) T *U) {
+__device__ __forceinline__ void apply_kind_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K],
+                                              int ukind, int swap,
+                                              const __attribute__((address_space(4))) T *U) {
     constexpr int D = 1 << K;
-    constexpr int dim = 1 << M;
     if (ukind == U_DIAG) {
 #pragma unroll
         for (int i = 0; i < D; i++) {
             const int r = op_index<M, QA, QB>(i);
-            v[i] = cmul(Cx<T>{U[2 * r], U[2 * r + 1]}, v[i]);
+            dst[i] = cmul(Cx<T>{U[2 * r], U[2 * r + 1]}, src[i]);
         }
     } else if (ukind == U_SWAP) {
         if constexpr (M == 1) {
-            swap_pair<T, K, M, QA, QB, 0, 1>(v);
+            permute_to<T, K, M, QA, QB, 0, 1>(dst, src);
         } else {
             switch (swap) {
-            case 1: swap_pair<T, K, M, QA, QB, 0, 1>(v); break;
-            case 2: swap_pair<T, K, M, QA, QB, 0, 2>(v); break;
-            case 3: swap_pair<T, K, M, QA, QB, 0, 3>(v); break;
-            case 6: swap_pair<T, K, M, QA, QB, 1, 2>(v); break;
-            case 7: swap_pair<T, K, M, QA, QB, 1, 3>(v); break;
-            default: swap_pair<T, K, M, QA, QB, 2, 3>(v); break;
+            case 1: permute_to<T, K, M, QA, QB, 0, 1>(dst, src); break;
+            case 2: permute_to<T, K, M, QA, QB, 0, 2>(dst, src); break;
+            case 3: permute_to<T, K, M, QA, QB, 0, 3>(dst, src); break;
+            case 6: permute_to<T, K, M, QA, QB, 1, 2>(dst, src); break;
+            case 7: permute_to<T, K, M, QA, QB, 1, 3>(dst, src); break;
+            default: permute_to<T, K, M, QA, QB, 2, 3>(dst, src); break;
             }
         }
     } else if (ukind == U_REAL) {
@@ -297,9 +296,9 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
                     continue;
                 }
                 const int j = i | (1 << QA);
-                const Cx<T> a = v[i], b = v[j];
-                v[i] = {fma(m00, a.r, m01 * b.r), fma(m00, a.i, m01 * b.i)};
-                v[j] = {fma(m10, a.r, m11 * b.r), fma(m10, a.i, m11 * b.i)};
+                const Cx<T> a = src[i], b = src[j];
+                dst[i] = {fma(m00, a.r, m01 * b.r), fma(m00, a.i, m01 * b.i)};
+                dst[j] = {fma(m10, a.r, m11 * b.r), fma(m10, a.i, m11 * b.i)};
             }
         } else {
 #pragma unroll
@@ -308,20 +307,15 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
                     continue;
                 }
                 const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
-                Cx<T> a[4];
 #pragma unroll
                 for (int r = 0; r < 4; r++) {
-                    a[r] = v[idx[r]];
-                }
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    T sr = U[4 * r] * a[0].r, si = U[4 * r] * a[0].i;
+                    T sr = U[4 * r] * src[idx[0]].r, si = U[4 * r] * src[idx[0]].i;
 #pragma unroll
                     for (int c = 1; c < 4; c++) {
-                        sr = fma(U[4 * r + c], a[c].r, sr);
-                        si = fma(U[4 * r + c], a[c].i, si);
+                        sr = fma(U[4 * r + c], src[idx[c]].r, sr);
+                        si = fma(U[4 * r + c], src[idx[c]].i, si);
                     }
-                    v[idx[r]] = {sr, si};
+                    dst[idx[r]] = {sr, si};
                 }
             }
         }
@@ -332,14 +326,14 @@ __device__ __forceinline__ void apply_kind(Cx<T> (&v)[1 << K], int ukind, int sw
             for (int e = 0; e < 4; e++) {
                 u[e] = {U[2 * e], U[2 * e + 1]};
             }
-            ap1<T, K, QA>(v, u);
+            gen1<T, K, QA>(dst, src, u);
         } else {
             Cx<T> u[16];
 #pragma unroll
             for (int e = 0; e < 16; e++) {
                 u[e] = {U[2 * e], U[2 * e + 1]};
             }
-            ap2<T, K, QA, QB>(v, u);
+            gen2<T, K, QA, QB>(dst, src, u);
         }
     }
 }
@@ -394,23 +388,74 @@ __device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1
     }
 }
 
+// One op of a group: (fs, hs) -> (fd, hd). For a trainable op, the FP64
+// inner products Im<hs_j | G fs> enter the register queue q (newest first).
+template <typename T, int K, int NJ>
+__device__ __forceinline__ void op_step(const Cx<T> (&fs)[1 << K],
+                                        const Cx<T> (&hs)[NJ > 0 ? NJ : 1][1 << K],
+                                        Cx<T> (&fd)[1 << K], Cx<T> (&hd)[NJ > 0 ? NJ : 1][1 << K],
+                                        double (&q)[SEG_PMAX][NJ > 0 ? NJ : 1], int &nt,
+                                        const __attribute__((address_space(4))) unsigned char *base,
+                                        int o) {
+    constexpr int D = 1 << K;
+    using CT = __attribute__((address_space(4))) const T;
+    const auto *heads = reinterpret_cast<const __attribute__((address_space(4))) OpHead *>(base);
+    const int cls = heads[o].cls;
+    const int slot = heads[o].slot;
+    const int kinds = heads[o].kinds;
+    const uint32_t offs = static_cast<uint32_t>(heads[o].offs);
+    const int ukind = kinds & 255;
+    const int gkind = (kinds >> 8) & 255;
+    const int swp = kinds >> 16;
+    CT *U = reinterpret_cast<CT *>(base + (offs & 0xffff));
+    CT *G = reinterpret_cast<CT *>(base + (offs >> 16));
+    dispatch_cls<K>(cls, [&](auto Mc, auto QAc, auto QBc) {
+        constexpr int m = decltype(Mc)::value;
+        constexpr int qa = decltype(QAc)::value;
+        constexpr int qb = decltype(QBc)::value;
+        if constexpr (NJ > 0) {
+            if (slot >= 0) {
+                Cx<T> gf[D];
+                gen_kind<T, K, m, qa, qb>(gf, fs, gkind, G);
+#pragma unroll
+                for (int j = 0; j < NJ; j++) {
+                    const double d = im_dot(hs[j], gf);
+#pragma unroll
+                    for (int s = SEG_PMAX - 1; s > 0; s--) {
+                        q[s][j] = q[s - 1][j];
+                    }
+                    q[0][j] = d;
+                }
+                nt++;
+            }
+        }
+        apply_kind_to<T, K, m, qa, qb>(fd, fs, ukind, swp, U);
+        if constexpr (NJ > 0) {
+#pragma unroll
+            for (int j = 0; j < NJ; j++) {
+                apply_kind_to<T, K, m, qa, qb>(hd[j], hs[j], ukind, swp, U);
+            }
+        }
+    });
+}
+
 // Applies nops fused operations (descriptors in device memory, read through
 // the constant address space so the uniform loads become scalar loads) to
 // the register group f of phi and the groups h[j] of NJ observable states.
-// For a trainable op the FP64 inner products Im<h_j | G f> are added to
-// acc[slot][j] before the op is applied. The trainable ops of a group hold
+// Ops alternate between (f, h) and a second register set, so each result
+// lands in registers whose old contents are dead and the loop carries no
+// copies. For a trainable op the FP64 inner products Im<h_j | G f> are added
+// to acc[slot][j] before the op is applied. The trainable ops of a group hold
 // slots 0, 1, 2, ... in op order, so their products enter a register queue
-// (newest first) and reach acc[slot] once per group, after the loop: the
-// loop body needs no selects over the slots.
+// (newest first) and reach acc[slot] once per group, after the loop: the loop
+// body needs no selects over the slots.
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? NJ : 1][1 << K],
                                         double (&acc)[SEG_PMAX][NJ > 0 ? NJ : 1],
                                         const void *group, int nops) {
     constexpr int D = 1 << K;
-    using CT = __attribute__((address_space(4))) const T;
-    const auto *base = (const __attribute__((address_space(4))) unsigned char *)(group);
-    const auto *heads = reinterpret_cast<const __attribute__((address_space(4))) OpHead *>(base);
     constexpr int NA = NJ > 0 ? NJ : 1;
+    const auto *base = (const __attribute__((address_space(4))) unsigned char *)(group);
     double q[SEG_PMAX][NA];
 #pragma unroll
     for (int s = 0; s < SEG_PMAX; s++) {
@@ -420,44 +465,22 @@ __device__ __forceinline__ void run_ops(Cx<T> (&f)[1 << K], Cx<T> (&h)[NJ > 0 ? 
         }
     }
     int nt = 0;
-    for (int o = 0; o < nops; o++) {
-        const int cls = heads[o].cls;
-        const int slot = heads[o].slot;
-        const int kinds = heads[o].kinds;
-        const uint32_t offs = static_cast<uint32_t>(heads[o].offs);
-        const int ukind = kinds & 255;
-        const int gkind = (kinds >> 8) & 255;
-        const int swp = kinds >> 16;
-        CT *U = reinterpret_cast<CT *>(base + (offs & 0xffff));
-        CT *G = reinterpret_cast<CT *>(base + (offs >> 16));
-        dispatch_cls<K>(cls, [&](auto Mc, auto QAc, auto QBc) {
-            constexpr int m = decltype(Mc)::value;
-            constexpr int qa = decltype(QAc)::value;
-            constexpr int qb = decltype(QBc)::value;
-            if constexpr (NJ > 0) {
-                if (slot >= 0) {
-                    Cx<T> gf[D];
-                    gen_kind<T, K, m, qa, qb>(gf, f, gkind, G);
+    Cx<T> fb[D];
+    Cx<T> hb[NA][D];
+    for (int o = 0; o < nops; o += 2) {
+        op_step<T, K, NJ>(f, h, fb, hb, q, nt, base, o);
+        if (o + 1 < nops) {
+            op_step<T, K, NJ>(fb, hb, f, h, q, nt, base, o + 1);
+        } else {
 #pragma unroll
-                    for (int j = 0; j < NJ; j++) {
-                        const double d = im_dot(h[j], gf);
-#pragma unroll
-                        for (int s = SEG_PMAX - 1; s > 0; s--) {
-                            q[s][j] = q[s - 1][j];
-                        }
-                        q[0][j] = d;
-                    }
-                    nt++;
-                }
-            }
-            apply_kind<T, K, m, qa, qb>(f, ukind, swp, U);
-            if constexpr (NJ > 0) {
+            for (int i = 0; i < D; i++) {
+                f[i] = fb[i];
 #pragma unroll
                 for (int j = 0; j < NJ; j++) {
-                    apply_kind<T, K, m, qa, qb>(h[j], ukind, swp, U);
+                    h[j][i] = hb[j][i];
                 }
             }
-        });
+        }
     }
     if constexpr (NJ > 0) {
         // Align the queue so that q[SEG_PMAX - 1 - s] holds slot s.
