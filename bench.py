@@ -190,7 +190,7 @@ def main():
                          "reference for HBM byte counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
+        print("bench.py measures one GPU: the engine has no multi-GPU path "
               "(sharding is design-only); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
 

@@ -1,6 +1,6 @@
 """Large-size cases checked through size-independent properties.
 
-- The 26-qubit (1 GiB) 2000-gate stream of SURVEY config C2 followed by its
+- A 26-qubit (1 GiB) stream of 2000 random gates followed by its
   inverse returns to |0...0>, and the norm after the forward stream is 1.
 - A 29-qubit complex128 state (8 GiB) exercises the kernels' 64-bit byte
   offsets (states above 4 GiB): same round trip.
