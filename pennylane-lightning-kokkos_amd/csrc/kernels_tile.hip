@@ -27,6 +27,19 @@ __device__ __forceinline__ uint32_t ins0(uint32_t k, int b) {
     return ((k >> b) << (b + 1)) | low;
 }
 
+constexpr int kBlockBits = 8;
+static_assert((1 << kBlockBits) == TILE_BLOCK, "tile offsets split at the block size");
+
+// Global offset of the tile positions above the block bits for pass k of
+// the gather / scatter loop (uniform across the block).
+__device__ __forceinline__ uint64_t tile_high_offset(const TileLaunch &L, int k) {
+    uint64_t off = 0;
+    for (int q = kBlockBits; q < L.T; q++) {
+        off |= static_cast<uint64_t>((k >> (q - kBlockBits)) & 1) << L.tb[q];
+    }
+    return off;
+}
+
 template <typename T, int K, int NJ>
 __device__ __forceinline__ void run_bundle(Cx<T> *__restrict__ tile, int TS,
                                            const TileBundle &bd, const unsigned char *ops,
@@ -127,6 +140,12 @@ __global__ void __launch_bounds__(TILE_BLOCK)
         sp[j + 1] = hp.p[j];
     }
     const uint64_t ntiles = uint64_t{1} << (L.n - L.T);
+    // Tile position t = threadIdx.x + TILE_BLOCK * k: the global offset of
+    // t's low bits is fixed per thread, that of k is uniform.
+    uint64_t off_thread = 0;
+    for (int q = 0; q < kBlockBits && q < L.T; q++) {
+        off_thread |= static_cast<uint64_t>((threadIdx.x >> q) & 1) << L.tb[q];
+    }
     const auto *cb = reinterpret_cast<const __attribute__((address_space(4))) TileBundle *>(
         (const __attribute__((address_space(4))) char *)(bundles));
     for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
@@ -134,11 +153,8 @@ __global__ void __launch_bounds__(TILE_BLOCK)
         for (int q = 0; q < L.T; q++) {
             base = insert_zero(base, L.tb[q]);
         }
-        for (int t = threadIdx.x; t < TS; t += TILE_BLOCK) {
-            uint64_t off = 0;
-            for (int q = 0; q < L.T; q++) {
-                off |= static_cast<uint64_t>((t >> q) & 1) << L.tb[q];
-            }
+        for (int t = threadIdx.x, k = 0; t < TS; t += TILE_BLOCK, k++) {
+            const uint64_t off = off_thread | tile_high_offset(L, k);
             const Off bo = static_cast<Off>((base + off) * sizeof(Cx<T>));
 #pragma unroll
             for (int s = 0; s < NS; s++) {
@@ -156,11 +172,8 @@ __global__ void __launch_bounds__(TILE_BLOCK)
             }
             __syncthreads();
         }
-        for (int t = threadIdx.x; t < TS; t += TILE_BLOCK) {
-            uint64_t off = 0;
-            for (int q = 0; q < L.T; q++) {
-                off |= static_cast<uint64_t>((t >> q) & 1) << L.tb[q];
-            }
+        for (int t = threadIdx.x, k = 0; t < TS; t += TILE_BLOCK, k++) {
+            const uint64_t off = off_thread | tile_high_offset(L, k);
             const Off bo = static_cast<Off>((base + off) * sizeof(Cx<T>));
             if (L.write_phi) {
                 sto(sp[0], bo, tile[t]);
