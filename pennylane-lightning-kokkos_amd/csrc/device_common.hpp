@@ -343,7 +343,6 @@ template <typename T, int K, int M, int QA, int QB>
 __device__ __forceinline__ void gen_kind(Cx<T> (&gf)[1 << K], const Cx<T> (&f)[1 << K], int gkind,
                                          const __attribute__((address_space(4))) T *G) {
     constexpr int D = 1 << K;
-    constexpr int dim = 1 << M;
     if (gkind == G_Z) {
 #pragma unroll
         for (int i = 0; i < D; i++) {

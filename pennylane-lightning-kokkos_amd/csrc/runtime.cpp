@@ -136,8 +136,8 @@ ProfScope::~ProfScope() {
         return;
     }
     if (hipEventRecord(e1, stream) != hipSuccess) {
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
         return;
     }
     std::lock_guard<std::mutex> lock(g_prof_mutex);
