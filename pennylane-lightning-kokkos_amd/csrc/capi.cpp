@@ -255,14 +255,6 @@ std::vector<Step> plan_steps(std::vector<PlanOp> &&ops) {
     return steps;
 }
 
-std::vector<int> op_bits(const PlanOp &p, int n) {
-    std::vector<int> b;
-    for (auto w : p.U.wires) {
-        b.push_back(n - 1 - static_cast<int>(w));
-    }
-    return b;
-}
-
 void prepare(PlanOp &p) {
     densify(p.U);
     if (p.trainable) {
