@@ -30,3 +30,14 @@ def test_batched_errors(capi):
                              [z], [[0]])
     ev, rows = capi.adjoint_batched([], [], [])
     assert len(ev) == 0 and rows == []
+
+
+def test_null_state_handles_are_rejected(capi):
+    import ctypes
+
+    L = capi.load()
+    out = (ctypes.c_double * 4)()
+    rc = L.plk_probs(None, None, 0, out)
+    assert rc != 0 and "null state handle" in L.plk_last_error().decode()
+    rc = L.plk_generate_samples(None, 1, 0, None)
+    assert rc != 0

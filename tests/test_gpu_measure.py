@@ -107,3 +107,13 @@ def test_generate_samples_module(dtype):
     smp = np.array(sv.GenerateSamples(2, 1000))
     assert smp.shape == (1000, 2) and set(np.unique(smp)) <= {0, 1}
     assert not smp[:, 1].any() and 350 < smp[:, 0].sum() < 650
+
+
+def test_probs_and_samples_argument_errors(capi):
+    s = capi.State(4)
+    with pytest.raises(capi.PLException, match="repeated wire"):
+        s.probs([1, 1])
+    with pytest.raises(capi.PLException, match="out of range"):
+        s.probs([4])
+    assert s.generate_samples(0).shape == (0, 4)
+    np.testing.assert_allclose(s.probs([3, 0]), [1, 0, 0, 0])
