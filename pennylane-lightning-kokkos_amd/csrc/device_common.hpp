@@ -102,6 +102,43 @@ __device__ __forceinline__ double row_total16(double v) {
     return v;
 }
 
+// Value of lane CTRL-partner within the same 16-lane row for the DPP
+// permutations row_mirror (0x140), row_half_mirror (0x141) and quad_perm
+// (0x00-0xFF), whose sources are always inside the row.
+template <int CTRL> __device__ __forceinline__ double dpp_partner(double v) {
+    const uint64_t b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_update_dpp(0, static_cast<int>(b & 0xffffffffu), CTRL, 0xF,
+                                               0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(0, static_cast<int>(b >> 32), CTRL, 0xF, 0xF,
+                                               false);
+    return __longlong_as_double(static_cast<long long>(
+        (static_cast<uint64_t>(static_cast<uint32_t>(hi)) << 32) | static_cast<uint32_t>(lo)));
+}
+
+template <int CTRL, int H>
+__device__ __forceinline__ void butterfly_step(double (&v)[16], bool upper) {
+#pragma unroll
+    for (int k = 0; k < H; k++) {
+        const double lo = v[k], hi = v[k + H];
+        v[k] = (upper ? hi : lo) + dpp_partner<CTRL>(upper ? lo : hi);
+    }
+}
+
+// Reduce-scatter of 16 values over each 16-lane row: afterwards lane r
+// (lane % 16) holds the row total of value r. Partners lane^15, ^7, ^3, ^1
+// (row_mirror, row_half_mirror, quad_perm [3,2,1,0], [1,0,3,2]); at each
+// step a lane keeps the half selected by its bit 3, 2, 1, 0. The tree is
+// fixed, so the sums are deterministic; 15 exchange-adds per lane instead of
+// 4 per value for a 16-lane scan of each value.
+__device__ __forceinline__ double row_reduce_scatter16(double (&v)[16]) {
+    const int r = static_cast<int>(__lane_id()) & 15;
+    butterfly_step<0x140, 8>(v, (r >> 3) & 1);
+    butterfly_step<0x141, 4>(v, (r >> 2) & 1);
+    butterfly_step<0x1B, 2>(v, (r >> 1) & 1);
+    butterfly_step<0xB1, 1>(v, r & 1);
+    return v[0];
+}
+
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {

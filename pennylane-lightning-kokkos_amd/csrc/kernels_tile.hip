@@ -98,7 +98,24 @@ __device__ __forceinline__ void run_bundle(Cx<T> *__restrict__ tile, int TS,
             }
         }
     }
-    if constexpr (NJ > 0) {
+    if constexpr (NJ > 0 && SEG_PMAX * NJ <= 16) {
+        if (bd.nslots > 0) {
+            // One partial per 16-lane row: lane r of the row ends with the row
+            // total of value r = SEG_PMAX slot-major (slot, observable) pairs.
+            double v[16];
+#pragma unroll
+            for (int k = 0; k < 16; k++) {
+                v[k] = k < SEG_PMAX * NJ ? acc[k / NA][k % NA] : 0.0;
+            }
+            const double tot = row_reduce_scatter16(v);
+            const int r = threadIdx.x & 15;
+            const int s = r / NJ;
+            if (r < SEG_PMAX * NJ && s < bd.nslots) {
+                const int row = threadIdx.x >> 4;
+                lacc[(row * pass_slots + bd.slot0 + s) * NJ + r % NJ] += tot;
+            }
+        }
+    } else if constexpr (NJ > 0) {
         if (bd.nslots > 0) {
             // One partial per 16-lane row; each row's last lane owns its slot,
             // so the accumulation order is fixed.
