@@ -4,6 +4,7 @@
 #include <map>
 #include <mutex>
 #include <unordered_map>
+#include <vector>
 
 #include "plk_internal.hpp"
 
@@ -152,20 +153,83 @@ void record_kernel(const char *family, float ms, double bytes) {
     s.bytes += bytes;
 }
 
+// Temporary states are recycled per device instead of being freed: every
+// kernel of the library runs on the device's one stream, so a buffer handed
+// out again is written only after the work that last used it. The pool holds
+// at most kPoolMaxBytes per device; an allocation that fails releases it.
+namespace {
+constexpr size_t kPoolMaxBytes = size_t{32} << 30;
+
+struct TmpPool {
+    std::multimap<size_t, void *> free;
+    size_t held = 0;
+};
+
+std::mutex g_pool_mutex;
+std::map<int, TmpPool> g_pools;
+
+void release_pool(int dev) {
+    std::vector<void *> bufs;
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        TmpPool &p = g_pools[dev];
+        for (auto &kv : p.free) {
+            bufs.push_back(kv.second);
+        }
+        p.free.clear();
+        p.held = 0;
+    }
+    if (!bufs.empty()) {
+        (void)hipDeviceSynchronize();
+        for (void *b : bufs) {
+            (void)hipFree(b);
+        }
+    }
+}
+} // namespace
+
 TmpState::TmpState(int n, int dtype, int dev) {
     s.n = n;
     s.dtype = dtype;
     s.len = size_t{1} << n;
     s.device = dev;
-    PLK_HIP(hipMalloc(&s.d, s.bytes()));
+    const size_t bytes = s.bytes();
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        TmpPool &p = g_pools[dev];
+        auto it = p.free.find(bytes);
+        if (it != p.free.end()) {
+            s.d = it->second;
+            p.held -= bytes;
+            p.free.erase(it);
+            return;
+        }
+    }
+    if (hipMalloc(&s.d, bytes) != hipSuccess) {
+        (void)hipGetLastError();
+        release_pool(dev);
+        s.d = nullptr;
+        PLK_HIP(hipMalloc(&s.d, bytes));
+    }
 }
 
 TmpState::~TmpState() {
-    if (s.d) {
-        // Kernels enqueued on this buffer must finish before it is released.
-        (void)hipDeviceSynchronize();
-        (void)hipFree(s.d);
+    if (!s.d) {
+        return;
     }
+    const size_t bytes = s.bytes();
+    {
+        std::lock_guard<std::mutex> lock(g_pool_mutex);
+        TmpPool &p = g_pools[s.device];
+        if (p.held + bytes <= kPoolMaxBytes) {
+            p.free.emplace(bytes, s.d);
+            p.held += bytes;
+            return;
+        }
+    }
+    // Kernels enqueued on this buffer must finish before it is released.
+    (void)hipDeviceSynchronize();
+    (void)hipFree(s.d);
 }
 
 } // namespace plk
