@@ -298,80 +298,113 @@ __device__ __forceinline__ void permute_to(Cx<T> (&dst)[1 << K], const Cx<T> (&s
     }
 }
 
-// dst = U src for one group, by the structure of U. src and dst are distinct
-// arrays, so no result aliases an input still to be read.
+// dst = U src for one group, one helper per structure of U. src and dst are
+// distinct arrays, so no result aliases an input still to be read.
 template <typename T, int K, int M, int QA, int QB>
-__device__ __forceinline__ void apply_kind_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K],
-                                              int ukind, int swap,
-                                              const __attribute__((address_space(4))) T *U) {
+__device__ __forceinline__ void diag_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K],
+                                        const __attribute__((address_space(4))) T *U) {
+#pragma unroll
+    for (int i = 0; i < (1 << K); i++) {
+        const int r = op_index<M, QA, QB>(i);
+        dst[i] = cmul(Cx<T>{U[2 * r], U[2 * r + 1]}, src[i]);
+    }
+}
+
+template <typename T, int K, int M, int QA, int QB>
+__device__ __forceinline__ void real_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K],
+                                        const __attribute__((address_space(4))) T *U) {
     constexpr int D = 1 << K;
-    if (ukind == U_DIAG) {
+    if constexpr (M == 1) {
+        const T m00 = U[0], m01 = U[1], m10 = U[2], m11 = U[3];
 #pragma unroll
         for (int i = 0; i < D; i++) {
-            const int r = op_index<M, QA, QB>(i);
-            dst[i] = cmul(Cx<T>{U[2 * r], U[2 * r + 1]}, src[i]);
+            if (i & (1 << QA)) {
+                continue;
+            }
+            const int j = i | (1 << QA);
+            const Cx<T> a = src[i], b = src[j];
+            dst[i] = {fma(m00, a.r, m01 * b.r), fma(m00, a.i, m01 * b.i)};
+            dst[j] = {fma(m10, a.r, m11 * b.r), fma(m10, a.i, m11 * b.i)};
         }
+    } else {
+#pragma unroll
+        for (int i = 0; i < D; i++) {
+            if (i & ((1 << QA) | (1 << QB))) {
+                continue;
+            }
+            const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                T sr = U[4 * r] * src[idx[0]].r, si = U[4 * r] * src[idx[0]].i;
+#pragma unroll
+                for (int c = 1; c < 4; c++) {
+                    sr = fma(U[4 * r + c], src[idx[c]].r, sr);
+                    si = fma(U[4 * r + c], src[idx[c]].i, si);
+                }
+                dst[idx[r]] = {sr, si};
+            }
+        }
+    }
+}
+
+template <typename T, int K, int M, int QA, int QB>
+__device__ __forceinline__ void dense_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K],
+                                         const __attribute__((address_space(4))) T *U) {
+    if constexpr (M == 1) {
+        Cx<T> u[4];
+#pragma unroll
+        for (int e = 0; e < 4; e++) {
+            u[e] = {U[2 * e], U[2 * e + 1]};
+        }
+        gen1<T, K, QA>(dst, src, u);
+    } else {
+        Cx<T> u[16];
+#pragma unroll
+        for (int e = 0; e < 16; e++) {
+            u[e] = {U[2 * e], U[2 * e + 1]};
+        }
+        gen2<T, K, QA, QB>(dst, src, u);
+    }
+}
+
+// (fd, hd[j]) = U (fs, hs[j]) for phi and the NJ observable states. The
+// structure of U is tested once per op and each branch applies it to every
+// state, so the states' independent FP64 chains share one basic block and
+// interleave in the schedule.
+template <typename T, int K, int M, int QA, int QB, int NJ>
+__device__ __forceinline__ void apply_kind_all(Cx<T> (&fd)[1 << K],
+                                               Cx<T> (&hd)[NJ > 0 ? NJ : 1][1 << K],
+                                               const Cx<T> (&fs)[1 << K],
+                                               const Cx<T> (&hs)[NJ > 0 ? NJ : 1][1 << K],
+                                               int ukind, int swap,
+                                               const __attribute__((address_space(4))) T *U) {
+    auto all = [&](auto &&fn) {
+        fn(fd, fs);
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            fn(hd[j], hs[j]);
+        }
+    };
+    using C = Cx<T>[1 << K];
+    if (ukind == U_DIAG) {
+        all([&](C &d, const C &x) { diag_to<T, K, M, QA, QB>(d, x, U); });
     } else if (ukind == U_SWAP) {
         if constexpr (M == 1) {
-            permute_to<T, K, M, QA, QB, 0, 1>(dst, src);
+            all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 0, 1>(d, x); });
         } else {
             switch (swap) {
-            case 1: permute_to<T, K, M, QA, QB, 0, 1>(dst, src); break;
-            case 2: permute_to<T, K, M, QA, QB, 0, 2>(dst, src); break;
-            case 3: permute_to<T, K, M, QA, QB, 0, 3>(dst, src); break;
-            case 6: permute_to<T, K, M, QA, QB, 1, 2>(dst, src); break;
-            case 7: permute_to<T, K, M, QA, QB, 1, 3>(dst, src); break;
-            default: permute_to<T, K, M, QA, QB, 2, 3>(dst, src); break;
+            case 1: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 0, 1>(d, x); }); break;
+            case 2: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 0, 2>(d, x); }); break;
+            case 3: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 0, 3>(d, x); }); break;
+            case 6: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 1, 2>(d, x); }); break;
+            case 7: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 1, 3>(d, x); }); break;
+            default: all([&](C &d, const C &x) { permute_to<T, K, M, QA, QB, 2, 3>(d, x); }); break;
             }
         }
     } else if (ukind == U_REAL) {
-        if constexpr (M == 1) {
-            const T m00 = U[0], m01 = U[1], m10 = U[2], m11 = U[3];
-#pragma unroll
-            for (int i = 0; i < D; i++) {
-                if (i & (1 << QA)) {
-                    continue;
-                }
-                const int j = i | (1 << QA);
-                const Cx<T> a = src[i], b = src[j];
-                dst[i] = {fma(m00, a.r, m01 * b.r), fma(m00, a.i, m01 * b.i)};
-                dst[j] = {fma(m10, a.r, m11 * b.r), fma(m10, a.i, m11 * b.i)};
-            }
-        } else {
-#pragma unroll
-            for (int i = 0; i < D; i++) {
-                if (i & ((1 << QA) | (1 << QB))) {
-                    continue;
-                }
-                const int idx[4] = {i, i | (1 << QB), i | (1 << QA), i | (1 << QA) | (1 << QB)};
-#pragma unroll
-                for (int r = 0; r < 4; r++) {
-                    T sr = U[4 * r] * src[idx[0]].r, si = U[4 * r] * src[idx[0]].i;
-#pragma unroll
-                    for (int c = 1; c < 4; c++) {
-                        sr = fma(U[4 * r + c], src[idx[c]].r, sr);
-                        si = fma(U[4 * r + c], src[idx[c]].i, si);
-                    }
-                    dst[idx[r]] = {sr, si};
-                }
-            }
-        }
+        all([&](C &d, const C &x) { real_to<T, K, M, QA, QB>(d, x, U); });
     } else {
-        if constexpr (M == 1) {
-            Cx<T> u[4];
-#pragma unroll
-            for (int e = 0; e < 4; e++) {
-                u[e] = {U[2 * e], U[2 * e + 1]};
-            }
-            gen1<T, K, QA>(dst, src, u);
-        } else {
-            Cx<T> u[16];
-#pragma unroll
-            for (int e = 0; e < 16; e++) {
-                u[e] = {U[2 * e], U[2 * e + 1]};
-            }
-            gen2<T, K, QA, QB>(dst, src, u);
-        }
+        all([&](C &d, const C &x) { dense_to<T, K, M, QA, QB>(d, x, U); });
     }
 }
 
@@ -465,13 +498,7 @@ __device__ __forceinline__ void op_step(const Cx<T> (&fs)[1 << K],
                 nt++;
             }
         }
-        apply_kind_to<T, K, m, qa, qb>(fd, fs, ukind, swp, U);
-        if constexpr (NJ > 0) {
-#pragma unroll
-            for (int j = 0; j < NJ; j++) {
-                apply_kind_to<T, K, m, qa, qb>(hd[j], hs[j], ukind, swp, U);
-            }
-        }
+        apply_kind_all<T, K, m, qa, qb, NJ>(fd, hd, fs, hs, ukind, swp, U);
     });
 }
 
