@@ -275,6 +275,24 @@ template <int M, int QA, int QB> __device__ __forceinline__ constexpr int op_ind
     return M == 1 ? ((i >> QA) & 1) : ((((i >> QA) & 1) << 1) | ((i >> QB) & 1));
 }
 
+// A register copy the compiler must keep where it is written. A permutation
+// writes its out-of-place result as plain copies of its source; left to
+// itself the compiler places those copies ahead of the branch on the gate
+// structure, so every other structure's path executes them and then
+// overwrites them. As instructions of their own they stay in the
+// permutation's branch.
+template <typename T> __device__ __forceinline__ Cx<T> kept_copy(Cx<T> x) {
+    Cx<T> y;
+    if constexpr (sizeof(T) == 8) {
+        asm("v_mov_b64 %0, %1" : "=v"(y.r) : "v"(x.r));
+        asm("v_mov_b64 %0, %1" : "=v"(y.i) : "v"(x.i));
+    } else {
+        asm("v_mov_b32 %0, %1" : "=v"(y.r) : "v"(x.r));
+        asm("v_mov_b32 %0, %1" : "=v"(y.i) : "v"(x.i));
+    }
+    return y;
+}
+
 // dst = src with the elements whose op-local indices are A and B exchanged.
 template <typename T, int K, int M, int QA, int QB, int A, int B>
 __device__ __forceinline__ void permute_to(Cx<T> (&dst)[1 << K], const Cx<T> (&src)[1 << K]) {
@@ -294,7 +312,7 @@ __device__ __forceinline__ void permute_to(Cx<T> (&dst)[1 << K], const Cx<T> (&s
                 }
             }
         }
-        dst[i] = src[j];
+        dst[i] = kept_copy(src[j]);
     }
 }
 
