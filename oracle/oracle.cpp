@@ -18,11 +18,17 @@
 //
 // Parity pins (checked by tests/test_oracle_*.py):
 //   * named gates: golden vectors transcribed from
-//     src/tests/Test_StateVectorKokkos_Param.cpp and _NonParam.cpp, plus
-//     U(theta) == expm(i * s * theta * G) built with scipy from this file's own
-//     generator matrices, unitarity and the inverse flag (U U^dagger = I);
-//   * generators: central finite differences of the gate
-//     (src/tests/Test_StateVectorKokkos_Generator.cpp) and the expm identity;
+//     src/tests/Test_StateVectorKokkos_Param.cpp (IsingXX/XY/YY/ZZ, MultiRZ,
+//     RX, RY, RZ, PhaseShift, ControlledPhaseShift, Rot, CRot, Single- and
+//     DoubleExcitation+-) and _NonParam.cpp (Hadamard, PauliX/Y/Z, S, T,
+//     CNOT, SWAP, CZ, Toffoli, CSWAP); every gate (CY, CRX, CRY, CRZ
+//     included) against a textbook matrix written in the test from its
+//     definition (controlled gates as |0><0| I + |1><1| U), and the inverse
+//     flag as U^dagger;
+//   * generators: numpy matrices of G as the reference's generator kernels
+//     apply it, with their scales, written in the test; U(theta) ==
+//     expm(i s theta G) from those two independent tables; central finite
+//     differences of the gate (src/tests/Test_StateVectorKokkos_Generator.cpp);
 //   * expectation values: goldens of src/tests/Test_StateVectorKokkos_Expval.cpp
 //     and tests/test_hamiltonian*.py, plus dense numpy brute force;
 //   * adjoint Jacobian: goldens of src/tests/Test_AdjointDiffKokkos.cpp, central

@@ -23,12 +23,33 @@ def _c(pairs):
     return np.ascontiguousarray(a[:, 0] + 1j * a[:, 1])
 
 
-@pytest.mark.parametrize("g", [g for g in GOLD if g["kind"] == "gate"],
-                         ids=lambda g: f"{g['gate']}-{g['wires']}-{g['params']}-inv{g['inverse']}")
+def _gid(g):
+    if g["kind"] == "gate_seq":
+        return "seq-" + "-".join(f"{o[0]}{o[1]}" for o in g["ops"])
+    return f"{g['gate']}-{g['wires']}-{g['params']}-inv{g['inverse']}"
+
+
+@pytest.mark.parametrize("g", [g for g in GOLD if g["kind"] in ("gate", "gate_seq")], ids=_gid)
 def test_gate_goldens(g):
     psi = _c(g["init"])
-    O.apply(psi, g["gate"], g["wires"], g["inverse"], g["params"])
-    np.testing.assert_allclose(psi, _c(g["expected"]), atol=2e-6, rtol=1.2e-5, err_msg=g["source"])
+    ops = g["ops"] if g["kind"] == "gate_seq" else [[g["gate"], g["wires"], g["params"], g["inverse"]]]
+    for name, wires, params, inv in ops:
+        O.apply(psi, name, wires, inv, params)
+    # Entries the reference test checks ("check"), or the whole vector.
+    idx = g.get("check", list(range(psi.size)))
+    np.testing.assert_allclose(psi[idx], _c(g["expected"])[idx], atol=2e-6, rtol=1.2e-5, err_msg=g["source"])
+
+
+def test_golden_coverage_of_gate_kernels():
+    """Every named gate of the reference test files NonParam / Param has at
+    least one transcribed golden."""
+    have = {g["gate"] for g in GOLD if g["kind"] == "gate"}
+    have |= {o[0] for g in GOLD if g["kind"] == "gate_seq" for o in g["ops"]}
+    want = {"Hadamard", "PauliX", "PauliY", "PauliZ", "S", "T", "CNOT", "SWAP", "CZ", "Toffoli", "CSWAP",
+            "RX", "RY", "RZ", "PhaseShift", "ControlledPhaseShift", "Rot", "CRot", "IsingXX", "IsingXY",
+            "IsingYY", "IsingZZ", "MultiRZ", "SingleExcitation", "SingleExcitationMinus",
+            "SingleExcitationPlus", "DoubleExcitation", "DoubleExcitationMinus", "DoubleExcitationPlus"}
+    assert want <= have, sorted(want - have)
 
 
 def test_expval_goldens():
