@@ -427,7 +427,7 @@ int tile_bits_adjoint(int n, int dtype) {
     if (env_int("PLK_TILES", 1) == 0) {
         return 0;
     }
-    const int T = std::min(env_int("PLK_TILE_BITS_ADJ", 10), n - 8);
+    const int T = std::min({env_int("PLK_TILE_BITS_ADJ", 10), n - 8, TILE_TMAX});
     return T >= low_bits(dtype) + 2 ? T : 0;
 }
 
@@ -435,7 +435,7 @@ int tile_bits_forward(int n, int dtype) {
     if (env_int("PLK_TILES", 1) == 0) {
         return 0;
     }
-    const int T = std::min(env_int("PLK_TILE_BITS_FWD", dtype == PLK_C64 ? 11 : 10), n - 8);
+    const int T = std::min({env_int("PLK_TILE_BITS_FWD", dtype == PLK_C64 ? 11 : 10), n - 8, TILE_TMAX});
     return T >= low_bits(dtype) + 2 ? T : 0;
 }
 
@@ -761,7 +761,8 @@ struct SegTable {
 };
 
 template <typename T>
-std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable &tab) {
+std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, int device,
+                                 SegTable &tab) {
     std::vector<unsigned char> table;
     std::vector<TileBundle> btab;
     std::vector<SegMeta> metas;
@@ -777,11 +778,11 @@ std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable
         }
     }
     if (!table.empty()) {
-        PLK_HIP(hipMalloc(&tab.d, table.size()));
+        tab.d = dev_malloc(device, table.size(), "operation table");
         PLK_HIP(hipMemcpy(tab.d, table.data(), table.size(), hipMemcpyHostToDevice));
     }
     if (!btab.empty()) {
-        PLK_HIP(hipMalloc(&tab.bundles, btab.size() * sizeof(TileBundle)));
+        tab.bundles = dev_malloc(device, btab.size() * sizeof(TileBundle), "bundle table");
         PLK_HIP(hipMemcpy(tab.bundles, btab.data(), btab.size() * sizeof(TileBundle),
                           hipMemcpyHostToDevice));
     }
@@ -790,8 +791,8 @@ std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, SegTable
 
 std::vector<SegMeta> upload_plan_any(const std::vector<Step> &steps, const State &s,
                                      SegTable &tab) {
-    return s.dtype == PLK_C64 ? upload_plan<float>(steps, s.n, tab)
-                              : upload_plan<double>(steps, s.n, tab);
+    return s.dtype == PLK_C64 ? upload_plan<float>(steps, s.n, s.device, tab)
+                              : upload_plan<double>(steps, s.n, s.device, tab);
 }
 
 void apply_single(Device &dev, const State &s, const CompiledOp &op);
@@ -1046,9 +1047,8 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
         hstate[j] = &H[j]->s;
     }
 
-    double *d_jac = nullptr;
-    PLK_HIP(hipMalloc(&d_jac, sizeof(double) * n_obs * n_tp));
-    PLK_HIP(hipMemsetAsync(d_jac, 0, sizeof(double) * n_obs * n_tp, dev.stream));
+    double *d_jac = ensure_jac(dev, n_obs * n_tp);
+    launch_fill_zero_doubles(dev, d_jac, n_obs * n_tp);
     std::vector<std::pair<size_t, double>> host_entries; // (flat index, value)
 
     SegTable tab;
@@ -1122,7 +1122,6 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     PLK_HIP(hipMemcpyAsync(jac, d_jac, sizeof(double) * n_obs * n_tp, hipMemcpyDeviceToHost,
                            dev.stream));
     PLK_HIP(hipStreamSynchronize(dev.stream));
-    PLK_HIP(hipFree(d_jac));
     for (auto &e : host_entries) {
         jac[e.first] = e.second;
     }
@@ -1480,6 +1479,7 @@ using namespace plk;
 
 #define PLK_TRY(body)                                                          \
     try {                                                                      \
+        std::lock_guard<std::recursive_mutex> plk_engine_lock_(engine_mutex()); \
         body;                                                                  \
         g_last_error.clear();                                                  \
         return PLK_OK;                                                         \
@@ -1554,7 +1554,7 @@ int plk_state_create(int num_qubits, int dtype, int device_id, plk_state **out) 
         h->s.dtype = dtype;
         h->s.len = size_t{1} << num_qubits;
         h->s.device = device_id;
-        PLK_HIP(hipMalloc(&h->s.d, h->s.bytes()));
+        h->s.d = dev_malloc(device_id, h->s.bytes(), "state vector");
         launch_set_basis(dev, h->s, 0);
         PLK_HIP(hipStreamSynchronize(dev.stream));
         *out = h.release();

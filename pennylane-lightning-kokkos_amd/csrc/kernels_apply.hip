@@ -221,6 +221,13 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+__global__ void __launch_bounds__(256) zero_doubles_kernel(double *__restrict__ p, uint64_t len) {
+    for (uint64_t i = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < len;
+         i += uint64_t{gridDim.x} * blockDim.x) {
+        p[i] = 0.0;
+    }
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256)
     scatter_kernel(Cx<T> *__restrict__ sv, const uint64_t *__restrict__ idx,
@@ -459,6 +466,16 @@ void launch_parity(Device &dev, const State &s, const std::vector<size_t> &wires
 
 void launch_fill_zero(Device &dev, const State &s) {
     launch_set_basis(dev, s, ~uint64_t{0});
+}
+
+void launch_fill_zero_doubles(Device &dev, double *p, size_t len) {
+    if (len == 0) {
+        return;
+    }
+    const unsigned grid = grid_for(dev, len);
+    hipLaunchKernelGGL(zero_doubles_kernel, dim3(grid), dim3(256), 0, dev.stream, p,
+                       static_cast<uint64_t>(len));
+    PLK_HIP(hipGetLastError());
 }
 
 void launch_set_basis(Device &dev, const State &s, size_t index) {

@@ -8,6 +8,7 @@
 #include <cstring>
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -51,10 +52,21 @@ struct Device {
     int num_cus = 256;
     void *work = nullptr;         // grow-only device workspace
     size_t work_cap = 0;          // bytes
+    double *jac = nullptr;        // grow-only Jacobian accumulator
+    size_t jac_cap = 0;           // doubles
 };
 
 Device &device(int id);
+// Serializes every entry point of the C ABI: the per-device scratch buffers
+// (partials, workspace, Jacobian), the temporary-state pool and the launch
+// caches are shared by all callers of the library.
+std::recursive_mutex &engine_mutex();
+// hipMalloc on the current device; on failure the device's pool of recycled
+// temporary states is released and the allocation retried once.
+void *dev_malloc(int device, size_t bytes, const char *what);
 void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles);
+// Grows d.jac to at least `doubles` entries.
+double *ensure_jac(Device &d, size_t doubles);
 // Grows d.work to at least `bytes` (synchronizes the stream when it grows).
 void *ensure_work(Device &d, size_t bytes);
 void set_device(int id);
@@ -294,6 +306,8 @@ void launch_parity(Device &dev, const State &s, const std::vector<size_t> &wires
                    cd even, cd odd);
 
 void launch_fill_zero(Device &dev, const State &s);
+// Zeroes len doubles with a kernel (no memset node when a caller captures the stream).
+void launch_fill_zero_doubles(Device &dev, double *p, size_t len);
 void launch_set_basis(Device &dev, const State &s, size_t index);
 void launch_scatter(Device &dev, const State &s, const size_t *d_idx,
                     const void *d_vals, size_t count);

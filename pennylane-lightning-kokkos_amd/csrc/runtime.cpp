@@ -24,6 +24,11 @@ std::mutex g_dev_mutex;
 std::map<int, std::unique_ptr<Device>> g_devices;
 } // namespace
 
+std::recursive_mutex &engine_mutex() {
+    static std::recursive_mutex m;
+    return m;
+}
+
 void set_device(int id) { PLK_HIP(hipSetDevice(id)); }
 
 Device &device(int id) {
@@ -59,7 +64,7 @@ void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles) {
             PLK_HIP(hipFree(d.red));
             d.red = nullptr;
         }
-        PLK_HIP(hipMalloc(&d.red, partial_doubles * sizeof(double)));
+        d.red = static_cast<double *>(dev_malloc(d.id, partial_doubles * sizeof(double), "partials"));
         d.red_cap = partial_doubles;
     }
     if (out_doubles > d.red_out_cap) {
@@ -68,7 +73,7 @@ void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles) {
             PLK_HIP(hipFree(d.red_out));
             d.red_out = nullptr;
         }
-        PLK_HIP(hipMalloc(&d.red_out, out_doubles * sizeof(double)));
+        d.red_out = static_cast<double *>(dev_malloc(d.id, out_doubles * sizeof(double), "reductions"));
         d.red_out_cap = out_doubles;
     }
 }
@@ -81,13 +86,25 @@ void *ensure_work(Device &d, size_t bytes) {
             d.work = nullptr;
             d.work_cap = 0;
         }
-        if (hipMalloc(&d.work, bytes) != hipSuccess) {
-            (void)hipGetLastError();
-            raise(PLK_ERR_OOM, "device workspace of " + std::to_string(bytes) + " bytes");
-        }
+        d.work = dev_malloc(d.id, bytes, "workspace");
         d.work_cap = bytes;
     }
     return d.work;
+}
+
+double *ensure_jac(Device &d, size_t doubles) {
+    if (doubles > d.jac_cap) {
+        PLK_HIP(hipStreamSynchronize(d.stream));
+        if (d.jac) {
+            PLK_HIP(hipFree(d.jac));
+            d.jac = nullptr;
+            d.jac_cap = 0;
+        }
+        const size_t cap = std::max<size_t>(doubles, 4096);
+        d.jac = static_cast<double *>(dev_malloc(d.id, cap * sizeof(double), "Jacobian"));
+        d.jac_cap = cap;
+    }
+    return d.jac;
 }
 
 // ---------------------------------------------------------------------------
@@ -188,6 +205,20 @@ void release_pool(int dev) {
 }
 } // namespace
 
+void *dev_malloc(int device, size_t bytes, const char *what) {
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes) == hipSuccess) {
+        return p;
+    }
+    (void)hipGetLastError();
+    release_pool(device);
+    if (hipMalloc(&p, bytes) == hipSuccess) {
+        return p;
+    }
+    (void)hipGetLastError();
+    raise(PLK_ERR_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed (" + what + ")");
+}
+
 TmpState::TmpState(int n, int dtype, int dev) {
     s.n = n;
     s.dtype = dtype;
@@ -205,12 +236,7 @@ TmpState::TmpState(int n, int dtype, int dev) {
             return;
         }
     }
-    if (hipMalloc(&s.d, bytes) != hipSuccess) {
-        (void)hipGetLastError();
-        release_pool(dev);
-        s.d = nullptr;
-        PLK_HIP(hipMalloc(&s.d, bytes));
-    }
+    s.d = dev_malloc(dev, bytes, "temporary state");
 }
 
 TmpState::~TmpState() {

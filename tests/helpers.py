@@ -174,6 +174,63 @@ def c1_workload(seed=20261202):
     return n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs
 
 
+def c2_workload(n=26, count=2000, seed=20261203, ham_terms=64):
+    """Gate-throughput stream: `count` gates drawn uniformly from
+    C2_GATES on uniform distinct wires, angles uniform in [0, 2 pi); then a
+    Hamiltonian of `ham_terms` random Pauli words of weight 1-4 with N(0, 1)
+    coefficients."""
+    rng = np.random.default_rng(seed)
+    ops = random_ops(n, count, rng, names=C2_GATES, allow_inverse=False)
+    ham = random_hamiltonian(n, ham_terms, rng)
+    return n, ops, ham
+
+
+C2_GATES = ["Hadamard", "RX", "RY", "RZ", "CNOT", "CZ", "CRX", "SWAP", "IsingXX", "IsingZZ"]
+
+
+def pauli_words(n, ham):
+    """(x mask, z mask, coefficient) per term of a Hamiltonian of Pauli words;
+    wire w is bit n - 1 - w of the amplitude index."""
+    out = []
+    for c, t in zip(ham[1], ham[2]):
+        factors = [t] if t[0] == "named" else t[1]
+        x = z = 0
+        ny = 0
+        for f in factors:
+            b = 1 << (n - 1 - f[2][0])
+            if f[1] in ("PauliX", "PauliY"):
+                x |= b
+            if f[1] in ("PauliZ", "PauliY"):
+                z |= b
+            ny += f[1] == "PauliY"
+        out.append((x, z, complex(c) * 1j ** ny))  # Y = i X Z on one wire
+    return out
+
+
+def pauli_csr(n, ham, index_dtype=np.int64):
+    """CSR (data, indices, indptr) of a sum of Pauli words, built directly:
+    the word c X^x Z^z (Y = i X Z per wire, folded into c) has, in row r,
+    the single entry c (-1)^popcount(col & z) at column col = r ^ x. Rows
+    hold one entry per distinct x mask, columns sorted. Numpy only, so
+    20-qubit matrices build in seconds."""
+    N = 1 << n
+    rows = np.arange(N, dtype=np.int64)
+    by_x = {}
+    for x, z, c in pauli_words(n, ham):
+        # (X^x Z^z)|col> = (-1)^popcount(col & z) |col ^ x>
+        col = rows ^ x
+        sign = 1.0 - 2.0 * (np.bitwise_count(col & z) & 1)
+        by_x[x] = by_x.get(x, 0) + c * sign
+    xs = sorted(by_x)
+    cols = np.stack([rows ^ x for x in xs], axis=1)
+    vals = np.stack([by_x[x] for x in xs], axis=1).astype(np.complex128)
+    order = np.argsort(cols, axis=1, kind="stable")
+    cols = np.take_along_axis(cols, order, axis=1)
+    vals = np.take_along_axis(vals, order, axis=1)
+    indptr = np.arange(0, N * len(xs) + 1, len(xs), dtype=index_dtype)
+    return vals.reshape(-1), cols.reshape(-1).astype(index_dtype), indptr
+
+
 def c3_workload(n=24, layers=10, seed=20261204, ham_terms=32):
     """Strongly entangling layers: Rot(phi, theta, omega) on every wire as
     RZ, RY, RZ, then CNOT(i, (i + r) mod n) with r = (layer mod (n-1)) + 1.
