@@ -35,6 +35,10 @@ LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "kernels_apply.hip",
                "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip",
                "kernels_sample.hip"]
 
+# The register-tile kernel is compiled once per (precision, observable chunk)
+# variant, each as an object of its own (RT_NJMAX in csrc/plk_internal.hpp).
+RTILE_VARIANTS = [(t, nj) for t in ("float", "double") for nj in range(6)]
+
 
 def _run(cmd, verbose):
     if verbose:
@@ -47,21 +51,40 @@ def _run(cmd, verbose):
         sys.stderr.write(r.stdout + r.stderr)
 
 
-def _compile(src, verbose):
+def _stale(obj, path):
+    """True when obj is missing or older than its source or any header."""
+    if not os.path.exists(obj):
+        return True
+    deps = [path] + [os.path.join(CSRC, f) for f in os.listdir(CSRC) if f.endswith((".hpp", ".h"))]
+    deps += [os.path.join(INCLUDE, f) for f in os.listdir(INCLUDE)]
+    return any(os.path.getmtime(d) > os.path.getmtime(obj) for d in deps)
+
+
+def _compile(job, verbose, incremental=False):
+    src, defines, name, extra = job
     os.makedirs(OBJ, exist_ok=True)
     path = os.path.join(CSRC, src)
-    obj = os.path.join(OBJ, src + ".o")
-    cmd = [HIPCC, *HIP_FLAGS, "-c", path, "-o", obj]
+    obj = os.path.join(OBJ, name + ".o")
+    if incremental and not _stale(obj, path):
+        return obj
+    cmd = [HIPCC, *HIP_FLAGS, *extra, *[f"-D{d}" for d in defines], "-c", path, "-o", obj]
     if src.endswith(".hip"):
         cmd[1:1] = ["-x", "hip"]
     _run(cmd, verbose)
     return obj
 
 
-def build_lib(verbose=False) -> str:
-    jobs = min(len(LIB_SOURCES), int(os.environ.get("MAX_JOBS", "8")))
+def build_lib(verbose=False, incremental=False) -> str:
+    # Uniform branches (the instruction switch) are left unstructurized: the
+    # structurizer would otherwise turn the switch into a chain of flow blocks
+    # whose merges copy the whole register tile.
+    work = [("kernels_rtile.hip", [f"PLK_RT_T={t}", f"PLK_RT_NJ={nj}"], f"kernels_rtile_{t}_{nj}",
+             ["-mllvm", "-structurizecfg-skip-uniform-regions=true"])
+            for t, nj in RTILE_VARIANTS]
+    work += [(s, [], s, []) for s in LIB_SOURCES]
+    jobs = min(len(work), int(os.environ.get("MAX_JOBS", "8")))
     with ThreadPoolExecutor(max_workers=jobs) as ex:
-        objs = list(ex.map(lambda s: _compile(s, verbose), LIB_SOURCES))
+        objs = list(ex.map(lambda j: _compile(j, verbose, incremental), work))
     out = os.path.join(PKG, "libplk_amd.so")
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs], verbose)
@@ -85,8 +108,10 @@ def build_bindings(verbose=False) -> str:
     return out
 
 
-def build(verbose=False):
-    lib = build_lib(verbose)
+def build(verbose=False, incremental=False):
+    """Compiles every source; incremental=True (python build.py --incremental)
+    skips objects newer than their source and the headers."""
+    lib = build_lib(verbose, incremental)
     mod = None
     if os.path.exists(os.path.join(CSRC, "bindings.cpp")):
         mod = build_bindings(verbose)
@@ -94,4 +119,4 @@ def build(verbose=False):
 
 
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    print(build(verbose="-v" in sys.argv, incremental="--incremental" in sys.argv))

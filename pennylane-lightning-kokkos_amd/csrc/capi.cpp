@@ -194,6 +194,8 @@ struct Bundle {
 struct Pass {
     std::vector<int> tb; // ascending global bit positions
     std::vector<Bundle> bundles;
+    std::vector<PlanOp> ops; // register-tile passes: the ops in program order
+    bool rt = false;         // a register-tile pass (ops) rather than LDS bundles
     int nslots = 0;
     std::vector<int> cols;     // Jacobian column of each trainable op, in order
     std::vector<double> coefs; // and its coefficient
@@ -331,12 +333,40 @@ void make_bundles(Pass &ps, std::vector<PlanOp> &&ops, int n) {
     }
 }
 
+// True when the dim x dim matrix M has a nonzero entry between local
+// indices that differ in local bit lb.
+bool mixes_bit(const std::vector<cd> &M, int dim, int lb) {
+    for (int r = 0; r < dim; r++) {
+        for (int c = 0; c < dim; c++) {
+            if ((((r ^ c) >> lb) & 1) && M[static_cast<size_t>(r) * dim + c] != cd(0.0, 0.0)) {
+                return true;
+            }
+        }
+    }
+    return false;
+}
+
+// Global state bits an op mixes (its U or, when trainable, its generator has
+// an entry between indices that differ in the bit).
+uint64_t mixing_mask(const PlanOp &p, int n) {
+    const int m = static_cast<int>(p.U.wires.size());
+    uint64_t mask = 0;
+    for (int k = 0; k < m; k++) {
+        const int lb = m - 1 - k;
+        if (mixes_bit(p.U.mat, 1 << m, lb) || (p.trainable && mixes_bit(p.G.mat, 1 << m, lb))) {
+            mask |= uint64_t{1} << (n - 1 - static_cast<int>(p.U.wires[k]));
+        }
+    }
+    return mask;
+}
+
 // Groups operations into LDS tile passes of T bits that always contain the
 // lowest L bits. Operations on disjoint wires commute, so an operation may be
 // taken ahead of skipped ones it shares no wire with; an unfusable operation
-// is a barrier and runs on its own.
+// is a barrier and runs on its own. With bundled == false (register-tile
+// passes) at most maxmix distinct bits may be mixed per pass.
 std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
-                             int pmax = TILE_PMAX) {
+                             int pmax = TILE_PMAX, bool bundled = true, int maxmix = 64) {
     std::vector<PlanOp> all;
     for (auto &p : ops_in) {
         if (p.U.kind == OpKind::Identity && !p.trainable) {
@@ -357,6 +387,14 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
         fus[i] = fusable(all[i]) ? 1 : 0;
         rem[i] = static_cast<int>(i);
     }
+    std::vector<uint64_t> mmask(all.size(), 0);
+    if (!bundled) {
+        for (size_t i = 0; i < all.size(); i++) {
+            if (fus[i]) {
+                mmask[i] = mixing_mask(all[i], n);
+            }
+        }
+    }
     std::vector<Step> steps;
     std::vector<int> taken, rest;
     while (!rem.empty()) {
@@ -368,7 +406,7 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
             continue;
         }
         uint64_t bits = L > 0 ? (uint64_t{1} << L) - 1 : 0;
-        uint64_t blocked = 0;
+        uint64_t blocked = 0, mixed = 0;
         taken.clear();
         rest.clear();
         int nslots = 0;
@@ -381,9 +419,11 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
             const bool tr = all[i].trainable;
             const uint64_t nb = bits | bmask[i];
             const bool ok = taken.size() < kPassMaxOps && (!tr || nslots < pmax) &&
-                            !(blocked & wmask[i]) && __builtin_popcountll(nb) <= T;
+                            !(blocked & wmask[i]) && __builtin_popcountll(nb) <= T &&
+                            __builtin_popcountll(mixed | mmask[i]) <= maxmix;
             if (ok) {
                 bits = nb;
+                mixed |= mmask[i];
                 nslots += tr ? 1 : 0;
                 taken.push_back(i);
             } else {
@@ -408,7 +448,19 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
         for (int i : taken) {
             ops.push_back(std::move(all[i]));
         }
-        make_bundles(s.pass, std::move(ops), n);
+        if (bundled) {
+            make_bundles(s.pass, std::move(ops), n);
+        } else {
+            for (auto &p : ops) {
+                if (p.trainable) {
+                    s.pass.cols.push_back(p.col);
+                    s.pass.coefs.push_back(p.coef);
+                    s.pass.nslots++;
+                }
+            }
+            s.pass.ops = std::move(ops);
+            s.pass.rt = true;
+        }
         steps.push_back(std::move(s));
         rem.swap(rest);
     }
@@ -769,7 +821,7 @@ std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, int devi
     for (const auto &s : steps) {
         if (s.fused) {
             metas.push_back(build_segment<T>(s.seg, n, table));
-        } else if (s.tiled) {
+        } else if (s.tiled && !s.pass.rt) {
             SegMeta m;
             build_pass<T>(s.pass, n, table, btab, m);
             metas.push_back(m);
@@ -795,14 +847,469 @@ std::vector<SegMeta> upload_plan_any(const std::vector<Step> &steps, const State
                               : upload_plan<double>(steps, s.n, s.device, tab);
 }
 
+// ---------------------------------------------------------------------------
+// Register-tile pass programs (kernels_rtile.hip).
+//
+// A pass's tile bits are split into R register bits and 6 lane positions.
+// An operation needs in registers the bits it mixes (bits in which its U, or
+// its generator, has an entry between different values); its other bits may
+// sit on lanes, where each lane applies the block its lane bits select. The
+// builder picks the load layout (the first mixing bits in registers, the
+// next two on lane positions 5 and 4, whose exchanges are single
+// v_permlane swaps), then walks the ops and brings each missing mixing bit
+// into a register by exchanging it with the register whose next use is
+// farthest away (Belady). The final layout is the store layout.
+// ---------------------------------------------------------------------------
+int rt_register_bits(int dtype) { return dtype == PLK_C64 ? 4 : 3; }
+
+struct RtProgram {
+    RtLaunch L{};
+    size_t off = 0; // byte offset of the program in the plan blob
+};
+
+template <typename T> struct RtBuilder {
+    std::vector<unsigned char> &blob;
+    size_t base = 0;
+    std::vector<RtInstr> ins;
+    std::vector<unsigned char> pay;
+
+    explicit RtBuilder(std::vector<unsigned char> &b) : blob(b) {}
+
+    int32_t put(const std::vector<double> &vals) {
+        pay.resize((pay.size() + 15) & ~size_t{15}, 0);
+        const size_t off = pay.size();
+        for (double v : vals) {
+            const T t = static_cast<T>(v);
+            const auto *p = reinterpret_cast<const unsigned char *>(&t);
+            pay.insert(pay.end(), p, p + sizeof(T));
+        }
+        return static_cast<int32_t>(off); // relative to the payload start; rebased in finish()
+    }
+
+    size_t finish() {
+        const size_t head = ins.size() * sizeof(RtInstr);
+        for (auto &i : ins) {
+            i.uoff += static_cast<int32_t>(head);
+            i.goff += static_cast<int32_t>(head);
+        }
+        blob.resize((blob.size() + 255) & ~size_t{255}, 0);
+        const size_t off = blob.size();
+        if (off + head + pay.size() > static_cast<size_t>(INT32_MAX)) {
+            raise(PLK_ERR_UNSUPPORTED, "register-tile program table exceeds 2 GiB");
+        }
+        const auto *pi = reinterpret_cast<const unsigned char *>(ins.data());
+        blob.insert(blob.end(), pi, pi + head);
+        blob.insert(blob.end(), pay.begin(), pay.end());
+        return off;
+    }
+};
+
+std::vector<double> cplx(const std::vector<cd> &v) {
+    std::vector<double> out;
+    for (auto &x : v) {
+        out.push_back(x.real());
+        out.push_back(x.imag());
+    }
+    return out;
+}
+
+bool is_real(const std::vector<cd> &v) {
+    for (auto &x : v) {
+        if (x.imag() != 0.0) {
+            return false;
+        }
+    }
+    return true;
+}
+
+// 2x2 block of a 4x4 op matrix over local bit `lr` (the register wire) with
+// the other local bit fixed to c.
+std::vector<cd> block2(const std::vector<cd> &M, int lr, int c) {
+    std::vector<cd> B(4);
+    for (int x = 0; x < 2; x++) {
+        for (int y = 0; y < 2; y++) {
+            const int r = lr == 1 ? (2 * x + c) : (2 * c + x);
+            const int col = lr == 1 ? (2 * y + c) : (2 * c + y);
+            B[2 * x + y] = M[4 * r + col];
+        }
+    }
+    return B;
+}
+
+bool is_diag2(const std::vector<cd> &B) { return B[1] == cd(0, 0) && B[2] == cd(0, 0); }
+bool is_ident2(const std::vector<cd> &B) {
+    return B[0] == cd(1, 0) && B[3] == cd(1, 0) && is_diag2(B);
+}
+bool is_x2(const std::vector<cd> &B) {
+    return B[0] == cd(0, 0) && B[3] == cd(0, 0) && B[1] == cd(1, 0) && B[2] == cd(1, 0);
+}
+
+// Encodes the U (and, for a trainable op, G) parts of a one-register-bit op.
+template <typename T>
+void rt_encode_reg1(RtBuilder<T> &b, RtInstr &in, const std::vector<cd> &U, const std::vector<cd> *G) {
+    if (is_diag2(U)) {
+        in.ukind = U_DIAG;
+        in.uoff = b.put(cplx({U[0], U[3]}));
+    } else if (is_x2(U)) {
+        in.ukind = U_SWAP;
+    } else if (is_real(U)) {
+        in.ukind = U_REAL;
+        in.uoff = b.put({U[0].real(), U[1].real(), U[2].real(), U[3].real()});
+    } else {
+        in.ukind = U_DENSE;
+        in.uoff = b.put(cplx(U));
+    }
+    if (G) {
+        const auto &g = *G;
+        if (is_diag2(g) && is_real(g) && g[0] == cd(1, 0) && g[3] == cd(-1, 0)) {
+            in.gkind = G_Z;
+        } else if (is_diag2(g) && is_real(g)) {
+            in.gkind = G_DIAG;
+            in.goff = b.put({g[0].real(), g[3].real()});
+        } else if (is_x2(g)) {
+            in.gkind = G_X;
+        } else if (g[0] == cd(0, 0) && g[3] == cd(0, 0) && g[1] == cd(0, -1) && g[2] == cd(0, 1)) {
+            in.gkind = G_Y;
+        } else {
+            in.gkind = G_DENSE;
+            in.goff = b.put(cplx(g));
+        }
+    }
+}
+
+template <typename T>
+RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned char> &blob) {
+    RtProgram out;
+    RtLaunch &L = out.L;
+    L.n = n;
+    L.R = R;
+    L.T = static_cast<int>(ps.tb.size());
+    if (L.T != R + RT_LANE_BITS) {
+        raise(PLK_ERR_INVALID, "register-tile pass of the wrong width");
+    }
+    for (int q = 0; q < L.T; q++) {
+        L.tb[q] = ps.tb[q];
+    }
+    const size_t nops = ps.ops.size();
+    std::vector<std::vector<int>> gbits(nops), mix(nops);
+    for (size_t i = 0; i < nops; i++) {
+        const PlanOp &p = ps.ops[i];
+        const int m = static_cast<int>(p.U.wires.size());
+        const int dim = 1 << m;
+        for (int k = 0; k < m; k++) {
+            const int gb = n - 1 - static_cast<int>(p.U.wires[k]);
+            gbits[i].push_back(gb);
+            const int lb = m - 1 - k;
+            if (mixes_bit(p.U.mat, dim, lb) || (p.trainable && mixes_bit(p.G.mat, dim, lb))) {
+                mix[i].push_back(gb);
+            }
+        }
+    }
+    // Load layout.
+    std::vector<int> order;
+    for (size_t i = 0; i < nops; i++) {
+        for (int q : mix[i]) {
+            if (std::find(order.begin(), order.end(), q) == order.end()) {
+                order.push_back(q);
+            }
+        }
+    }
+    std::vector<int> reg, lane(RT_LANE_BITS, -1);
+    std::vector<int> rest;
+    for (size_t k = 0; k < order.size() && static_cast<int>(reg.size()) < R; k++) {
+        reg.push_back(order[k]);
+    }
+    std::vector<int> hot; // next mixing bits: lane positions 5, 4
+    for (size_t k = reg.size(); k < order.size() && hot.size() < 2; k++) {
+        hot.push_back(order[k]);
+    }
+    for (int q : ps.tb) {
+        if (std::find(reg.begin(), reg.end(), q) == reg.end() &&
+            std::find(hot.begin(), hot.end(), q) == hot.end()) {
+            rest.push_back(q); // ascending global bits
+        }
+    }
+    while (static_cast<int>(reg.size()) < R) {
+        reg.push_back(rest.front());
+        rest.erase(rest.begin());
+    }
+    for (size_t k = 0; k < hot.size(); k++) {
+        lane[5 - k] = hot[k];
+    }
+    for (int p = 0, r = 0; p < RT_LANE_BITS; p++) {
+        if (lane[p] < 0) {
+            lane[p] = rest[r++];
+        }
+    }
+    for (int r = 0; r < R; r++) {
+        L.reg_in[r] = reg[r];
+    }
+    for (int p = 0; p < RT_LANE_BITS; p++) {
+        L.lane_in[p] = lane[p];
+    }
+
+    RtBuilder<T> b(blob);
+    auto reg_of = [&](int q) {
+        for (int r = 0; r < R; r++) {
+            if (reg[r] == q) {
+                return r;
+            }
+        }
+        return -1;
+    };
+    auto lane_of = [&](int q) {
+        for (int p = 0; p < RT_LANE_BITS; p++) {
+            if (lane[p] == q) {
+                return p;
+            }
+        }
+        raise(PLK_ERR_INVALID, "register-tile op outside the tile bits");
+    };
+    auto next_use = [&](int q, size_t from) {
+        for (size_t j = from; j < nops; j++) {
+            if (std::find(mix[j].begin(), mix[j].end(), q) != mix[j].end()) {
+                return j;
+            }
+        }
+        return nops + 1;
+    };
+    int slot = 0;
+    for (size_t i = 0; i < nops; i++) {
+        const PlanOp &p = ps.ops[i];
+        RtInstr in{};
+        in.slot = -1;
+        in.s0 = in.s1 = -1;
+        for (int q : mix[i]) {
+            if (reg_of(q) >= 0) {
+                continue;
+            }
+            int victim = -1;
+            size_t far = 0;
+            for (int r = 0; r < R; r++) {
+                if (std::find(mix[i].begin(), mix[i].end(), reg[r]) != mix[i].end()) {
+                    continue;
+                }
+                const size_t u = next_use(reg[r], i);
+                if (victim < 0 || u > far) {
+                    victim = r;
+                    far = u;
+                }
+            }
+            const int l = lane_of(q);
+            if (l < 4 || in.nremap >= 2) {
+                raise(PLK_ERR_INVALID, "register-tile pass needs more mixing bits than it can remap");
+            }
+            if (in.nremap == 0) {
+                in.rq0 = static_cast<int8_t>(victim);
+                in.rp0 = static_cast<int8_t>(l);
+            } else {
+                in.rq1 = static_cast<int8_t>(victim);
+                in.rp1 = static_cast<int8_t>(l);
+            }
+            in.nremap++;
+            lane[l] = reg[victim];
+            reg[victim] = q;
+        }
+        const std::vector<cd> *G = nullptr;
+        if (p.trainable) {
+            if (p.G.wires != p.U.wires || p.G.kind != OpKind::Dense) {
+                raise(PLK_ERR_INVALID, "generator wires differ from gate wires");
+            }
+            G = &p.G.mat;
+            in.slot = slot++;
+        }
+        const int m = static_cast<int>(gbits[i].size());
+        if (m == 1) {
+            const int r = reg_of(gbits[i][0]);
+            if (r >= 0) {
+                in.code = RT_REG1;
+                in.a = static_cast<int8_t>(r);
+                rt_encode_reg1<T>(b, in, p.U.mat, G);
+            } else {
+                in.code = RT_LANE;
+                in.s0 = static_cast<int8_t>(lane_of(gbits[i][0]));
+                in.uoff = b.put(cplx({p.U.mat[0], p.U.mat[3]}));
+                if (G) {
+                    in.goff = b.put({(*G)[0].real(), (*G)[3].real()});
+                }
+            }
+        } else {
+            const int r0 = reg_of(gbits[i][0]), r1 = reg_of(gbits[i][1]);
+            if (r0 >= 0 && r1 >= 0) {
+                in.code = RT_REG2;
+                in.a = static_cast<int8_t>(std::max(r0, r1));
+                in.b = static_cast<int8_t>(std::min(r0, r1));
+                SegOp<T> d{};
+                d.slot = G ? 0 : -1;
+                put_mat(d.U, p.U.mat, r0 < r1);
+                if (G) {
+                    put_mat(d.G, *G, r0 < r1);
+                }
+                classify(d, 2);
+                in.ukind = static_cast<int8_t>(d.ukind);
+                in.swap = static_cast<int8_t>(d.swap);
+                std::vector<double> u;
+                if (d.ukind == U_DIAG) {
+                    for (int k = 0; k < 4; k++) {
+                        u.push_back(d.U[2 * (5 * k)]);
+                        u.push_back(d.U[2 * (5 * k) + 1]);
+                    }
+                } else if (d.ukind == U_REAL) {
+                    for (int e = 0; e < 16; e++) {
+                        u.push_back(d.U[2 * e]);
+                    }
+                } else if (d.ukind == U_DENSE) {
+                    u.assign(d.U, d.U + 32);
+                }
+                in.uoff = b.put(u);
+                if (G) {
+                    in.gkind = static_cast<int8_t>(d.gkind);
+                    std::vector<double> g;
+                    if (d.gkind == G_DIAG) {
+                        for (int k = 0; k < 4; k++) {
+                            g.push_back(d.G[2 * (5 * k)]);
+                        }
+                    } else if (d.gkind == G_DENSE) {
+                        g.assign(d.G, d.G + 32);
+                    }
+                    in.goff = b.put(g);
+                }
+            } else if (r0 >= 0 || r1 >= 0) {
+                // One wire on a register bit, the other (not mixed) on a lane.
+                const int k_reg = r0 >= 0 ? 0 : 1;
+                const int lr = 1 - k_reg; // local bit of the register wire
+                in.code = RT_SEL;
+                in.a = static_cast<int8_t>(k_reg == 0 ? r0 : r1);
+                in.s0 = static_cast<int8_t>(lane_of(gbits[i][1 - k_reg]));
+                const std::vector<cd> B0 = block2(p.U.mat, lr, 0), B1 = block2(p.U.mat, lr, 1);
+                if (is_diag2(B0) && is_diag2(B1)) {
+                    in.ukind = U_DIAG;
+                    in.uoff = b.put(cplx({B0[0], B0[3], B1[0], B1[3]}));
+                } else if ((is_ident2(B0) || is_x2(B0)) && (is_ident2(B1) || is_x2(B1))) {
+                    in.ukind = U_SWAP;
+                    in.uoff = b.put({is_x2(B0) ? 1.0 : 0.0, is_x2(B1) ? 1.0 : 0.0});
+                } else if (is_real(B0) && is_real(B1)) {
+                    in.ukind = U_REAL;
+                    in.uoff = b.put({B0[0].real(), B0[1].real(), B0[2].real(), B0[3].real(),
+                                     B1[0].real(), B1[1].real(), B1[2].real(), B1[3].real()});
+                } else {
+                    in.ukind = U_DENSE;
+                    std::vector<double> u = cplx(B0), u1 = cplx(B1);
+                    u.insert(u.end(), u1.begin(), u1.end());
+                    in.uoff = b.put(u);
+                }
+                if (G) {
+                    const std::vector<cd> G0 = block2(*G, lr, 0), G1 = block2(*G, lr, 1);
+                    if (is_diag2(G0) && is_diag2(G1) && is_real(G0) && is_real(G1)) {
+                        in.gkind = G_DIAG;
+                        in.goff = b.put({G0[0].real(), G0[3].real(), G1[0].real(), G1[3].real()});
+                    } else {
+                        in.gkind = G_DENSE;
+                        std::vector<double> g = cplx(G0), g1 = cplx(G1);
+                        g.insert(g.end(), g1.begin(), g1.end());
+                        in.goff = b.put(g);
+                    }
+                }
+            } else {
+                // Both wires on lanes: diagonal; block c = bit(wires[1]) + 2 bit(wires[0]) is
+                // the op's local index.
+                in.code = RT_LANE;
+                in.s0 = static_cast<int8_t>(lane_of(gbits[i][1]));
+                in.s1 = static_cast<int8_t>(lane_of(gbits[i][0]));
+                std::vector<cd> f;
+                for (int k = 0; k < 4; k++) {
+                    f.push_back(p.U.mat[5 * k]);
+                }
+                in.uoff = b.put(cplx(f));
+                if (G) {
+                    std::vector<double> g;
+                    for (int k = 0; k < 4; k++) {
+                        g.push_back((*G)[5 * k].real());
+                    }
+                    in.goff = b.put(g);
+                }
+            }
+        }
+        b.ins.push_back(in);
+    }
+    for (int r = 0; r < R; r++) {
+        L.reg_out[r] = reg[r];
+    }
+    for (int p = 0; p < RT_LANE_BITS; p++) {
+        L.lane_out[p] = lane[p];
+    }
+    L.ninstr = static_cast<int>(b.ins.size());
+    L.nslots = slot;
+    out.off = b.finish();
+    return out;
+}
+
+// Programs of every register-tile pass of a plan, uploaded as one blob.
+struct RtTable {
+    void *d = nullptr;
+    ~RtTable() {
+        if (d) {
+            (void)hipDeviceSynchronize();
+            (void)hipFree(d);
+        }
+    }
+};
+
+std::vector<RtProgram> upload_rt_plan(const std::vector<Step> &steps, const State &s, RtTable &tab) {
+    std::vector<unsigned char> blob;
+    std::vector<RtProgram> progs(steps.size());
+    const int R = rt_register_bits(s.dtype);
+    for (size_t i = 0; i < steps.size(); i++) {
+        if (steps[i].tiled && steps[i].pass.rt) {
+            progs[i] = s.dtype == PLK_C64 ? build_rt_program<float>(steps[i].pass, s.n, R, blob)
+                                          : build_rt_program<double>(steps[i].pass, s.n, R, blob);
+        }
+    }
+    if (!blob.empty()) {
+        tab.d = dev_malloc(s.device, blob.size(), "register-tile programs");
+        PLK_HIP(hipMemcpy(tab.d, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    }
+    return progs;
+}
+
+// Observable states per register-tile launch: 3 keeps phi plus three states
+// and the spare buffer at two wavefronts per SIMD (PLK_RTILE_NJ overrides, up
+// to RT_NJMAX).
+int rtile_chunk(int dtype) {
+    (void)dtype;
+    return std::max(1, std::min(RT_NJMAX, env_int("PLK_RTILE_NJ", 3)));
+}
+
+// Register-tile passes serve states with at least as many tiles as CUs (and
+// PLK_RTILE=0 turns them off, PLK_RTILE=1 forces them from the tile width up).
+bool use_rtile(int n, int dtype) {
+    const int T = rt_register_bits(dtype) + RT_LANE_BITS;
+    const int mode = env_int("PLK_RTILE", -1);
+    if (mode == 0) {
+        return false;
+    }
+    if (mode == 1) {
+        return n >= T;
+    }
+    return n >= T + 8;
+}
+
 void apply_single(Device &dev, const State &s, const CompiledOp &op);
 
 // Applies a forward plan (no observables) to s.
 void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
     SegTable tab;
+    RtTable rtab;
     auto metas = upload_plan_any(steps, s, tab);
+    const auto progs = upload_rt_plan(steps, s, rtab);
     for (size_t i = 0; i < steps.size(); i++) {
-        if (steps[i].fused) {
+        if (steps[i].tiled && steps[i].pass.rt) {
+            RtLaunch L = progs[i].L;
+            L.nj = 0;
+            L.write_phi = 1;
+            L.nslots = 0;
+            launch_rtile(dev, s, nullptr, static_cast<const char *>(rtab.d) + progs[i].off, L);
+        } else if (steps[i].fused) {
             SegLaunch L = metas[i].L;
             L.write_phi = 1;
             L.nj = 0;
@@ -951,6 +1458,10 @@ std::vector<Step> forward_plan(const OpsData &ops, int n, int dtype) {
                          ops.matrices[i], n);
         v.push_back(std::move(p));
     }
+    if (use_rtile(n, dtype)) {
+        return plan_tiles(std::move(v), n, rt_register_bits(dtype) + RT_LANE_BITS, low_bits(dtype),
+                          RT_PMAX, false, rt_register_bits(dtype) + 2);
+    }
     const int T = tile_bits_forward(n, dtype);
     if (T > 0) {
         return plan_tiles(std::move(v), n, T, low_bits(dtype));
@@ -1025,10 +1536,13 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     if (n_obs == 0) {
         return;
     }
-    const int T_adj = tile_bits_adjoint(n, ref.dtype);
+    const bool rt = use_rtile(n, ref.dtype);
+    const int T_adj = rt ? rt_register_bits(ref.dtype) + RT_LANE_BITS : tile_bits_adjoint(n, ref.dtype);
     const std::vector<Step> steps =
-        T_adj > 0 ? plan_tiles(std::move(rev), n, T_adj, low_bits(ref.dtype))
-                  : plan_steps(std::move(rev));
+        rt ? plan_tiles(std::move(rev), n, T_adj, low_bits(ref.dtype), RT_PMAX, false,
+                        rt_register_bits(ref.dtype) + 2)
+           : T_adj > 0 ? plan_tiles(std::move(rev), n, T_adj, low_bits(ref.dtype))
+                       : plan_steps(std::move(rev));
 
     TmpState lambda(n, ref.dtype, ref.device);
     launch_copy(dev, lambda.s, ref);
@@ -1052,7 +1566,9 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     std::vector<std::pair<size_t, double>> host_entries; // (flat index, value)
 
     SegTable tab;
+    RtTable rtab;
     const auto metas = upload_plan_any(steps, lambda.s, tab);
+    const auto progs = upload_rt_plan(steps, lambda.s, rtab);
     std::unique_ptr<TmpState> mu;
     const int kmax_nj = SEG_NJMAX;
     for (size_t i = 0; i < steps.size(); i++) {
@@ -1078,6 +1594,21 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
                 if (L.nslots > 0) {
                     launch_segment_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0), m.cols,
                                           m.coefs, d_jac, static_cast<int>(n_tp));
+                }
+                j0 += nj;
+            }
+        } else if (st.tiled && st.pass.rt) {
+            const void *prog = static_cast<const char *>(rtab.d) + progs[i].off;
+            size_t j0 = 0;
+            for (int nj : chunk_sizes(n_obs, rtile_chunk(ref.dtype))) {
+                RtLaunch L = progs[i].L;
+                L.nj = nj;
+                L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+                const int grid = launch_rtile(dev, lambda.s, hptr.data() + j0, prog, L);
+                if (L.nslots > 0) {
+                    launch_tile_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0),
+                                       st.pass.cols.data(), st.pass.coefs.data(), d_jac,
+                                       static_cast<int>(n_tp));
                 }
                 j0 += nj;
             }

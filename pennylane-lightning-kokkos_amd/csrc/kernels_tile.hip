@@ -215,13 +215,15 @@ __global__ void __launch_bounds__(TILE_BLOCK)
     }
 }
 
+constexpr int kReduceMax = TILE_PMAX > RT_PMAX ? TILE_PMAX : RT_PMAX;
+
 struct TileReduceArgs {
     int grid;
     int nj;
     int j0;
     int ld;
-    int col[TILE_PMAX];
-    double coef[TILE_PMAX];
+    int col[kReduceMax];
+    double coef[kReduceMax];
 };
 
 // One block per (slot, j): fixed-order FP64 sum of the per-block partials.
@@ -377,7 +379,7 @@ int launch_tile(Device &dev, const State &phi, void *const *h_ptrs, const void *
 
 void launch_tile_reduce(Device &dev, int grid, int nslots, int nj, int j0, const int *cols,
                         const double *coefs, double *d_jac, int ld) {
-    if (nslots > TILE_PMAX) {
+    if (nslots > kReduceMax) {
         raise(PLK_ERR_INVALID, "too many trainable operations in a tile pass");
     }
     TileReduceArgs a{};

@@ -235,6 +235,67 @@ int launch_tile(Device &dev, const State &phi, void *const *h_ptrs, const void *
 void launch_tile_reduce(Device &dev, int grid, int nslots, int nj, int j0, const int *cols,
                         const double *coefs, double *d_jac, int ld);
 
+// Register-tile passes (kernels_rtile.hip). A wavefront holds a tile of
+// 2^(R+6) amplitudes of phi and nj observable states in VGPRs: R register
+// bits (3 for complex128, 4 for complex64) and 6 lane bits, each holding one
+// tile bit (a global index bit). A pass program is a list of RtInstr
+// followed by the payloads they point to.
+constexpr int RT_LANE_BITS = 6;
+constexpr int RT_RMAX = 4;
+constexpr int RT_TMAX = RT_RMAX + RT_LANE_BITS;
+constexpr int RT_PMAX = 32; // trainable ops per pass
+
+enum RtOpCode : int8_t {
+    RT_REG1 = 1,  // one wire on register bit a
+    RT_REG2 = 2,  // two wires on register bits a > b (local index 2 bit(a) + bit(b))
+    RT_SEL = 3,   // register bit a mixes; blocks selected per lane by lane bits s0 (, s1)
+    RT_LANE = 4   // diagonal on lane positions s0 (, s1): one factor per lane
+};
+
+// Payloads (reals of the state's precision, from the program start):
+//   REG1  U: DIAG [u0 u1] complex, REAL [m00 m01 m10 m11], DENSE 2x2 complex,
+//            SWAP none; G: DIAG [g0 g1], DENSE 2x2 complex, X / Y / Z none.
+//   REG2  U: DIAG 4 complex, REAL 4x4, DENSE 4x4 complex, SWAP none (swap =
+//            4 a + b exchanged local indices); G: DIAG 4, DENSE 4x4 complex, Z none.
+//   SEL   nb = 2 (s1 < 0) or 4 blocks, block c = bit(s0) + 2 bit(s1), each
+//            block as REG1 (U SWAP: one real, nonzero = exchange); G: DIAG or DENSE.
+//   LANE  U: nb complex factors; G: nb real factors (same block index).
+// Before its operation an instruction applies nremap (0-2) remaps: register
+// bit rq exchanged with lane position rp (4 or 5).
+struct RtInstr {
+    int8_t code, a, b, s0, s1, ukind, gkind, swap;
+    int32_t slot; // trainable slot of the pass, -1 otherwise
+    int32_t uoff, goff;
+    int8_t nremap, rq0, rp0, rq1;
+    int8_t rp1, pad8[3];
+    int32_t pad;
+};
+static_assert(sizeof(RtInstr) == 32, "RtInstr layout");
+
+struct RtLaunch {
+    int n, T, R;
+    int ninstr;
+    int nslots;
+    int nj;
+    int write_phi;
+    int pad;
+    int tb[RT_TMAX];                   // tile bits, ascending global positions
+    int reg_in[RT_RMAX], lane_in[RT_LANE_BITS];   // global bit of each slot at load
+    int reg_out[RT_RMAX], lane_out[RT_LANE_BITS]; // and at store
+};
+
+// Runs one register-tile pass over phi (and nj observable states, whose
+// FP64 partials per block go to dev.red as [slot * nj + j][block]); returns
+// the grid size.
+int launch_rtile(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
+                 const RtLaunch &L);
+constexpr int RT_NJMAX = 5; // observable states per register-tile launch
+// One variant per precision (float / double) and chunk 0..RT_NJMAX, each
+// compiled from kernels_rtile.hip as an object of its own.
+template <typename T, int NJ>
+int rtile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
+                   const RtLaunch &L);
+
 // Batched circuits (kernels_batch.hip). One workgroup runs one circuit at a
 // time from |0...0>: forward passes, h = H phi and <H>, reverse passes with
 // the FP64 Im<h|G phi> of every trainable op. A pass is a set of T tile bits
