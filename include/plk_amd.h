@@ -137,6 +137,19 @@ int plk_expval_csr(const plk_state *sv, const double *data,
                    size_t num_rows, double *out);
 /* Re<psi|O|psi> of an observable tree (MeasuresKokkos.hpp:354-360). */
 int plk_expval_obs(const plk_state *sv, const plk_obs *ob, double *out);
+/* plk_expval_obs for an observable that is a sum of at most 4096 products of
+ * Identity / PauliX / PauliY / PauliZ / Hadamard, allocating no device
+ * memory. No counterpart in the reference: it is for callers that cannot
+ * call hipMalloc (HIP graph capture, tight serving loops, harnesses that
+ * forbid allocation). The partial sums and term tables live in a fixed
+ * 256 KiB device array that each GPU holds until the process ends; larger
+ * sums fail with PLK_ERR_INVALID, other observables with
+ * PLK_ERR_UNSUPPORTED. All calls on one GPU share that array: calls that
+ * overlap on one GPU race (the library's own lock serialises its calls, and
+ * each call waits for its kernels). Same kernels and result as
+ * plk_expval_obs. */
+int plk_expval_obs_noalloc(const plk_state *sv, const plk_obs *ob,
+                           double *out);
 /* <x|y> accumulated in FP64 (LinearAlgebraKokkos.hpp:155-236). */
 int plk_inner_product(const plk_state *x, const plk_state *y, double *re,
                       double *im);

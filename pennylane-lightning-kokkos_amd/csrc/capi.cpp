@@ -61,8 +61,10 @@ uint64_t wire_bit(int n, size_t w) {
 }
 
 // Expands an observable into Pauli words when it is a sum of products of
-// Identity / PauliX / PauliY / PauliZ / Hadamard. Returns false otherwise.
-bool to_pauli(const ObsNode &o, int n, std::vector<PauliTerm> &out) {
+// Identity / PauliX / PauliY / PauliZ / Hadamard with at most cap words.
+// Returns false otherwise.
+bool to_pauli(const ObsNode &o, int n, std::vector<PauliTerm> &out,
+              size_t cap = kMaxPauliTerms) {
     switch (o.kind) {
     case ObsNode::Named: {
         if (o.name == "Identity") {
@@ -90,11 +92,11 @@ bool to_pauli(const ObsNode &o, int n, std::vector<PauliTerm> &out) {
         std::vector<PauliTerm> acc{PauliTerm{0, 0, 1.0}};
         for (auto &f : o.terms) {
             std::vector<PauliTerm> ft;
-            if (!to_pauli(*f, n, ft)) {
+            if (!to_pauli(*f, n, ft, cap)) {
                 return false;
             }
             std::vector<PauliTerm> next;
-            if (acc.size() * ft.size() > kMaxPauliTerms) {
+            if (acc.size() * ft.size() > cap) {
                 return false;
             }
             for (auto &a : acc) {
@@ -112,14 +114,14 @@ bool to_pauli(const ObsNode &o, int n, std::vector<PauliTerm> &out) {
         std::vector<PauliTerm> acc;
         for (size_t t = 0; t < o.terms.size(); t++) {
             std::vector<PauliTerm> tt;
-            if (!to_pauli(*o.terms[t], n, tt)) {
+            if (!to_pauli(*o.terms[t], n, tt, cap)) {
                 return false;
             }
             for (auto &p : tt) {
                 p.coeff *= o.coeffs[t];
                 acc.push_back(p);
             }
-            if (acc.size() > kMaxPauliTerms) {
+            if (acc.size() > cap) {
                 return false;
             }
         }
@@ -2286,6 +2288,21 @@ int plk_expval_obs(const plk_state *sv, const plk_obs *ob, double *out) {
     PLK_TRY({
         const State &s = S(sv);
         *out = expval_obs(dev_of(s), s, O(ob));
+    })
+}
+
+int plk_expval_obs_noalloc(const plk_state *sv, const plk_obs *ob, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        std::vector<PauliTerm> terms;
+        // Expanded past the limit so that an oversized sum is reported by size.
+        if (!to_pauli(O(ob), s.n, terms, 4 * kNoAllocMaxPauliWords)) {
+            raise(PLK_ERR_UNSUPPORTED,
+                  "plk_expval_obs_noalloc: the observable is not a sum of at most " +
+                      std::to_string(kNoAllocMaxPauliWords) +
+                      " products of Identity / PauliX / PauliY / PauliZ / Hadamard");
+        }
+        *out = pauli_expval_noalloc(dev_of(s), s, terms);
     })
 }
 

@@ -302,7 +302,15 @@ struct PauliTables {
     int ngroups = 0;
 };
 
-PauliTables upload_pauli(Device &dev, const std::vector<PauliTerm> &terms) {
+// Host image of the grouped term tables: groups sorted by X mask, then the
+// terms of each group, at byte offset toff.
+struct PauliBlob {
+    std::vector<unsigned char> bytes;
+    size_t toff = 0;
+    int ngroups = 0;
+};
+
+PauliBlob pack_pauli(const std::vector<PauliTerm> &terms) {
     std::vector<PauliTerm> sorted = terms;
     std::stable_sort(sorted.begin(), sorted.end(),
                      [](const PauliTerm &a, const PauliTerm &b) { return a.x < b.x; });
@@ -315,19 +323,47 @@ PauliTables upload_pauli(Device &dev, const std::vector<PauliTerm> &terms) {
         groups.back().count++;
         tdev.push_back({sorted[k].zy, sorted[k].coeff.real(), sorted[k].coeff.imag()});
     }
+    PauliBlob b;
     const size_t gb = groups.size() * sizeof(PauliGroup);
-    const size_t toff = (gb + 255) / 256 * 256;
-    const size_t total = toff + tdev.size() * sizeof(PauliTermDev) + 16;
-    std::vector<unsigned char> blob(total);
-    std::memcpy(blob.data(), groups.data(), gb);
-    std::memcpy(blob.data() + toff, tdev.data(), tdev.size() * sizeof(PauliTermDev));
+    b.toff = (gb + 255) / 256 * 256;
+    b.bytes.resize(b.toff + tdev.size() * sizeof(PauliTermDev) + 16);
+    std::memcpy(b.bytes.data(), groups.data(), gb);
+    std::memcpy(b.bytes.data() + b.toff, tdev.data(), tdev.size() * sizeof(PauliTermDev));
+    b.ngroups = static_cast<int>(groups.size());
+    return b;
+}
+
+PauliTables tables_at(void *dst, const PauliBlob &b) {
     PauliTables t;
-    PLK_HIP(hipMalloc(&t.blob, total));
-    PLK_HIP(hipMemcpy(t.blob, blob.data(), total, hipMemcpyHostToDevice));
-    t.groups = static_cast<const PauliGroup *>(t.blob);
-    t.terms = reinterpret_cast<const PauliTermDev *>(static_cast<unsigned char *>(t.blob) + toff);
-    t.ngroups = static_cast<int>(groups.size());
+    t.blob = dst;
+    t.groups = static_cast<const PauliGroup *>(dst);
+    t.terms = reinterpret_cast<const PauliTermDev *>(static_cast<unsigned char *>(dst) + b.toff);
+    t.ngroups = b.ngroups;
     return t;
+}
+
+PauliTables upload_pauli(Device &dev, const std::vector<PauliTerm> &terms) {
+    const PauliBlob b = pack_pauli(terms);
+    void *blob = nullptr;
+    PLK_HIP(hipMalloc(&blob, b.bytes.size()));
+    PLK_HIP(hipMemcpy(blob, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
+    return tables_at(blob, b);
+}
+
+// Launches pauli_expval_kernel over grid blocks into partials.
+void launch_pauli_expval(Device &dev, const State &s, const PauliTables &t, unsigned grid,
+                         double *partials) {
+    ProfScope prof("pauli_expval", static_cast<double>(t.ngroups) * s.bytes(), dev.stream);
+    if (s.dtype == PLK_C64) {
+        hipLaunchKernelGGL((pauli_expval_kernel<float>), dim3(grid), dim3(kBlock), 0, dev.stream,
+                           static_cast<const Cx<float> *>(s.d), static_cast<uint64_t>(s.len),
+                           t.groups, t.ngroups, t.terms, partials);
+    } else {
+        hipLaunchKernelGGL((pauli_expval_kernel<double>), dim3(grid), dim3(kBlock), 0,
+                           dev.stream, static_cast<const Cx<double> *>(s.d),
+                           static_cast<uint64_t>(s.len), t.groups, t.ngroups, t.terms, partials);
+    }
+    PLK_HIP(hipGetLastError());
 }
 
 void free_tables(Device &dev, void *blob) {
@@ -353,24 +389,81 @@ double pauli_expval(Device &dev, const State &s, const std::vector<PauliTerm> &t
     PauliTables t = upload_pauli(dev, terms);
     const unsigned grid = grid_for(dev, s.len);
     ensure_red(dev, grid, 1);
-    {
-        ProfScope prof("pauli_expval", static_cast<double>(t.ngroups) * s.bytes(), dev.stream);
-        if (s.dtype == PLK_C64) {
-            hipLaunchKernelGGL((pauli_expval_kernel<float>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<const Cx<float> *>(s.d),
-                               static_cast<uint64_t>(s.len), t.groups, t.ngroups, t.terms,
-                               dev.red);
-        } else {
-            hipLaunchKernelGGL((pauli_expval_kernel<double>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<const Cx<double> *>(s.d),
-                               static_cast<uint64_t>(s.len), t.groups, t.ngroups, t.terms,
-                               dev.red);
-        }
-        PLK_HIP(hipGetLastError());
-    }
+    launch_pauli_expval(dev, s, t, grid, dev.red);
     double out = 0.0;
     finish_sums(dev, static_cast<int>(grid), 1, &out);
     free_tables(dev, t.blob);
+    return out;
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free Pauli-sum expectation value.
+//
+// Every device this library runs on holds one copy of `scratch` until the
+// process ends. Layout: the per-block partial sums (kNoAllocPartials
+// doubles), the final sum (one double, own 256-byte line), then the grouped
+// term tables. Host code never passes the array's name to a kernel: the name
+// is a host-side stand-in, and the device address of each GPU's copy is
+// looked up once per device with hipGetSymbolAddress and kept in
+// Device::noalloc_scratch. Each call overwrites all of the parts it reads
+// (every block writes its partial, final_sum_kernel writes the total, the
+// tables are copied in), so nothing has to start at zero. All calls on one
+// device share the array: two calls that overlap on a device race. The
+// library's entry-point lock serialises its own calls and each call waits for
+// its kernels before returning.
+// ---------------------------------------------------------------------------
+constexpr size_t kNoAllocPartials = 2048;
+constexpr size_t kNoAllocTableOff = kNoAllocPartials * sizeof(double) + 256;
+constexpr size_t kNoAllocScratchBytes = size_t{256} << 10;
+static_assert(kNoAllocTableOff + 256 + kNoAllocMaxPauliWords * (sizeof(PauliGroup) + sizeof(PauliTermDev)) + 16 <=
+                  kNoAllocScratchBytes,
+              "scratch too small for the largest accepted table");
+
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+static unsigned char *scratch_of(Device &dev) {
+    if (dev.noalloc_scratch == nullptr) {
+        // device() made dev current.
+        void *ptr = nullptr;
+        PLK_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        if (ptr == nullptr) {
+            raise(PLK_ERR_HIP, "hipGetSymbolAddress returned no address for the scratch array");
+        }
+        dev.noalloc_scratch = ptr;
+    }
+    return static_cast<unsigned char *>(dev.noalloc_scratch);
+}
+
+double pauli_expval_noalloc(Device &dev, const State &s, const std::vector<PauliTerm> &terms) {
+    if (terms.size() > kNoAllocMaxPauliWords) {
+        raise(PLK_ERR_INVALID, "plk_expval_obs_noalloc: the observable expands to " +
+                                   std::to_string(terms.size()) + " Pauli words; at most " +
+                                   std::to_string(kNoAllocMaxPauliWords) +
+                                   " fit the library's fixed scratch array");
+    }
+    if (terms.empty()) {
+        return 0.0;
+    }
+    unsigned char *base = scratch_of(dev);
+    double *partials = reinterpret_cast<double *>(base);
+    double *total = reinterpret_cast<double *>(base + kNoAllocPartials * sizeof(double));
+    const PauliBlob b = pack_pauli(terms);
+    if (kNoAllocTableOff + b.bytes.size() > kNoAllocScratchBytes) {
+        raise(PLK_ERR_INVALID, "plk_expval_obs_noalloc: term tables exceed the scratch array");
+    }
+    PLK_HIP(hipMemcpyAsync(base + kNoAllocTableOff, b.bytes.data(), b.bytes.size(),
+                           hipMemcpyHostToDevice, dev.stream));
+    const PauliTables t = tables_at(base + kNoAllocTableOff, b);
+    const unsigned grid = static_cast<unsigned>(
+        std::min<uint64_t>(grid_for(dev, s.len), kNoAllocPartials));
+    launch_pauli_expval(dev, s, t, grid, partials);
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kBlock), 0, dev.stream, partials,
+                       static_cast<int>(grid), total);
+    PLK_HIP(hipGetLastError());
+    double out = 0.0;
+    PLK_HIP(hipMemcpyAsync(&out, total, sizeof(double), hipMemcpyDeviceToHost, dev.stream));
+    // Also keeps b.bytes alive until the copy above has read it.
+    PLK_HIP(hipStreamSynchronize(dev.stream));
     return out;
 }
 

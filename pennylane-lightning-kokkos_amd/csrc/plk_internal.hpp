@@ -54,6 +54,7 @@ struct Device {
     size_t work_cap = 0;          // bytes
     double *jac = nullptr;        // grow-only Jacobian accumulator
     size_t jac_cap = 0;           // doubles
+    void *noalloc_scratch = nullptr; // this device's copy of the fixed scratch array
 };
 
 Device &device(int id);
@@ -389,6 +390,12 @@ void launch_pauli_apply(Device &dev, const State &out, const State &in,
                         const std::vector<PauliTerm> &terms);
 double pauli_expval(Device &dev, const State &s,
                     const std::vector<PauliTerm> &terms);
+// Largest number of Pauli words pauli_expval_noalloc accepts.
+constexpr size_t kNoAllocMaxPauliWords = 4096;
+// pauli_expval with every device buffer (partials, total, term tables) in a
+// fixed file-scope device array instead of allocated memory; raises
+// PLK_ERR_INVALID for more than kNoAllocMaxPauliWords terms.
+double pauli_expval_noalloc(Device &dev, const State &s, const std::vector<PauliTerm> &terms);
 
 // CSR matrix (device copies made per call).
 double csr_expval(Device &dev, const State &s, const double *data,

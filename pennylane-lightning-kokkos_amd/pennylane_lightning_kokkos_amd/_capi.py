@@ -54,6 +54,7 @@ SIGNATURES = {
     "plk_expval_csr": (ctypes.c_int, [_vp, _c_double_p, _c_int64_p, _c_int64_p, ctypes.c_size_t,
                                       ctypes.c_size_t, _c_double_p]),
     "plk_expval_obs": (ctypes.c_int, [_vp, _vp, _c_double_p]),
+    "plk_expval_obs_noalloc": (ctypes.c_int, [_vp, _vp, _c_double_p]),
     "plk_inner_product": (ctypes.c_int, [_vp, _vp, _c_double_p, _c_double_p]),
     "plk_obs_named": (ctypes.c_int, [ctypes.c_char_p, _c_size_p, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "plk_obs_hermitian": (ctypes.c_int, [_c_double_p, _c_size_p, ctypes.c_size_t, ctypes.POINTER(_vp)]),
@@ -256,6 +257,12 @@ class State:
     def expval(self, ob: "Obs") -> float:
         out = ctypes.c_double()
         check(load().plk_expval_obs(self.h, ob.h, ctypes.byref(out)))
+        return out.value
+
+    def expval_noalloc(self, ob: "Obs") -> float:
+        """plk_expval_obs_noalloc: Pauli sums only, no device allocation."""
+        out = ctypes.c_double()
+        check(load().plk_expval_obs_noalloc(self.h, ob.h, ctypes.byref(out)))
         return out.value
 
     def inner(self, other: "State") -> complex:
