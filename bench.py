@@ -90,25 +90,196 @@ def run_gpu(capi, n, ops, obs, dtype, steps, warmup, profile=True):
     return jac, dt, stats, len(tp)
 
 
+def host_info():
+    """CPU model and a measured host memory bandwidth (STREAM-style triad
+    a = b + s c over three 1 GiB float64 arrays with torch's CPU threads;
+    24 bytes per element counted, write-allocate not counted)."""
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    out = {"cpu_model": model, "logical_cpus": os.cpu_count()}
+    try:
+        import torch
+
+        n = 1 << 27
+        a, b, c = torch.empty(n, dtype=torch.float64), torch.rand(n, dtype=torch.float64), torch.rand(n, dtype=torch.float64)
+        torch.add(b, c, alpha=0.5, out=a)
+        reps = 5
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            torch.add(b, c, alpha=0.5, out=a)
+        dt = time.perf_counter() - t0
+        out["host_triad_GBps"] = round(24.0 * n * reps / dt / 1e9, 1)
+        out["host_triad_threads"] = torch.get_num_threads()
+        del a, b, c
+    except Exception as e:  # the baseline is reported without it
+        out["host_triad_error"] = str(e)
+    return out
+
+
 def cpu_baseline(n, ops, obs, sample_params):
-    """Oracle (reference algorithm, C++/OpenMP) on the last `sample_params`
-    trainable parameters of the same circuit: forward pass, observable
-    application and the reverse sweep until those parameters are consumed."""
+    """Oracle (reference algorithm, C++/OpenMP) on the same workload, timed in
+    three parts: the forward pass (apply every op to |0>), the observable
+    application (H_j lambda for every observable) and one adjoint_jacobian
+    call whose trainable set is the last `sample_params` parameters (forward +
+    observables + reverse sweep until they are consumed). The sweep's cost per
+    parameter is that call minus the first two parts, divided by
+    `sample_params`; `value` is the full 720-parameter Jacobian rate this
+    implies: n_params / (forward + observables + n_params * per-parameter)."""
     import oracle as O
     from helpers import to_oracle
 
     names, params, wires, inv, mats = ops
     npar = sum(1 for p in params if p)
     tp = list(range(npar - sample_params, npar))
-    psi = O.zero_state(n)
+    oops = O.Ops(names, params, wires, inv, mats)
+    oobs = [to_oracle(o) for o in obs]
     t0 = time.perf_counter()
-    O.adjoint_jacobian(psi, [to_oracle(o) for o in obs], O.Ops(names, params, wires, inv, mats), tp, True)
+    psi = O.apply_ops(O.zero_state(n), oops)
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for o in oobs:
+        O.obs_apply(psi, o)
+    t_obs = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.adjoint_jacobian(O.zero_state(n), oobs, oops, tp, True)
+    t_all = time.perf_counter() - t0
+    per_param = max(t_all - t_fwd - t_obs, 1e-9) / sample_params
+    full = t_fwd + t_obs + npar * per_param
+    info = host_info()
+    return {"value": round(npar / full, 3), "unit": "params/s", "cores": O.num_threads(), "kind": "port",
+            "sample": (f"oracle (reference algorithm restated in C++/OpenMP) on the same {n}-qubit circuit "
+                       f"and {len(obs)} observables: forward pass {t_fwd:.2f} s, observable application "
+                       f"{t_obs:.2f} s, adjoint_jacobian with the last {sample_params} of {npar} parameters "
+                       f"trainable {t_all:.2f} s; value extrapolates the sweep to all {npar} parameters"),
+            "forward_s": round(t_fwd, 3), "observables_s": round(t_obs, 3),
+            "sweep_s_per_param": round(per_param, 4),
+            "sample_params_per_s": round(sample_params / t_all, 3), **info}
+
+
+# Bytes each gate of the 26-qubit stream moves when applied alone, as the
+# reference's per-gate kernels touch the state: dense or all-diagonal gates
+# read and write every amplitude (2 S), controlled gates and SWAP half of
+# them (S), CZ a quarter (S / 2).
+C2_GATE_BYTES_IN_S = {"Hadamard": 2.0, "RX": 2.0, "RY": 2.0, "RZ": 2.0, "IsingXX": 2.0, "IsingZZ": 2.0,
+                      "CNOT": 1.0, "CRX": 1.0, "SWAP": 1.0, "CZ": 0.5}
+
+
+def _family(stats, names):
+    tot = {"launches": 0.0, "total_ms": 0.0, "bytes": 0.0}
+    for k in names:
+        if k in stats:
+            for f in tot:
+                tot[f] += stats[k][f]
+    return tot
+
+
+def _collect(capi):
+    out = {k: capi.profiling_get(k) for k in capi.profiling_names()}
+    capi.profiling_enable(False)
+    return out
+
+
+def run_c2(capi, reps=5):
+    """The gate-throughput workload: a 2000-gate stream on a 26-qubit
+    complex128 state (1 GiB), <H> of 64 random Pauli words at 26 qubits, and
+    the same construction's Hamiltonian at 20 qubits as a CSR matrix
+    (device-resident observable; complex128 with int64 indices stored as
+    int32, complex64 with the int32 serialisation). Kernel figures come from
+    HIP events on the library's stream; GB/s are algorithmic bytes (per kernel
+    family: what one launch must read and write) over kernel time."""
+    from helpers import c2_workload, pauli_csr, to_capi
+
+    n, ops, ham = c2_workload()
+    S = (1 << n) * 16
+    s = capi.State(n)
+    cops = capi.Ops(*ops)
+    s.apply_ops(cops)
+    s.synchronize()
+    s.reset()
+    capi.profiling_enable(True)
+    t0 = time.perf_counter()
+    s.apply_ops(cops)
+    s.synchronize()
     dt = time.perf_counter() - t0
-    return {"value": sample_params / dt, "unit": "params/s", "cores": O.num_threads(), "kind": "port",
-            "sample": (f"oracle adjoint_jacobian on the same {n}-qubit circuit and 9 observables, "
-                       f"trainable = last {sample_params} of {npar} parameters (forward pass + "
-                       f"observable application + reverse sweep until consumed), one run, "
-                       f"{dt:.1f} s")}
+    st = _collect(capi)
+    fam = _family(st, ["apply_rtile", "apply_tile", "apply_segment", "apply_dense", "apply_parity"])
+    unfused = sum(C2_GATE_BYTES_IN_S[g] for g in ops[0]) * S
+    stream = {"gates": len(ops[0]), "ms": round(1e3 * dt, 2), "passes": fam["launches"],
+              "gates_per_pass": round(len(ops[0]) / max(1.0, fam["launches"]), 1),
+              "kernel_ms": round(fam["total_ms"], 2),
+              "kernel_GBps": round(fam["bytes"] / max(fam["total_ms"], 1e-9) / 1e6, 1),
+              "kernel_frac_of_peak": round(fam["bytes"] / max(fam["total_ms"], 1e-9) / 1e6 / HBM_PEAK_GBS, 3),
+              "unfused_gate_bytes": unfused,
+              "unfused_equivalent_GBps": round(unfused / dt / 1e9, 1),
+              "gates_per_s": round(len(ops[0]) / dt, 1)}
+    log(f"bench: C2 stream {1e3 * dt:.1f} ms, {fam['launches']:.0f} passes, "
+        f"{stream['kernel_GBps']} GB/s in pass kernels")
+    h = to_capi(capi, ham)
+    s.expval(h)
+    capi.profiling_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        e = s.expval(h)
+    dt = (time.perf_counter() - t0) / reps
+    st = _collect(capi)["pauli_expval"]
+    k_ms = st["total_ms"] / st["launches"]
+    groups = st["bytes"] / st["launches"] / S
+    ham_line = {"words": len(ham[1]), "x_mask_groups": int(round(groups)), "call_ms": round(1e3 * dt, 3),
+                "kernel_ms": round(k_ms, 3),
+                "kernel_GBps": round(st["bytes"] / st["launches"] / (k_ms * 1e-3) / 1e9, 1),
+                "per_word_equivalent_GBps": round(len(ham[1]) * S / (k_ms * 1e-3) / 1e9, 1),
+                "value": e}
+    log(f"bench: C2 <H> 64 words: kernel {k_ms:.3f} ms ({ham_line['x_mask_groups']} X-mask groups)")
+    del s
+    n20 = 20
+    _, _, ham20 = c2_workload(n=n20)
+    d, i, p = pauli_csr(n20, ham20, index_dtype=np.int32)
+    rows = {}
+    for name, dt_ in (("complex128", np.complex128), ("complex64", np.complex64)):
+        s20 = capi.State(n20, dt_)
+        s20.upload((np.ones(1 << n20) / np.sqrt(1 << n20)).astype(dt_))
+        ob = capi.Obs.sparse(d, i, p, list(range(n20)))
+        s20.expval(ob)
+        capi.profiling_enable(True)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            s20.expval(ob)
+        dtc = (time.perf_counter() - t0) / reps
+        st = _collect(capi)["csr_expval"]
+        k_ms = st["total_ms"] / st["launches"]
+        gbps = st["bytes"] / st["launches"] / (k_ms * 1e-3) / 1e9
+        t0 = time.perf_counter()
+        s20.expval_csr(d, i, p)
+        once = time.perf_counter() - t0
+        rows[name] = {"kernel_ms": round(k_ms, 4), "call_ms": round(1e3 * dtc, 3),
+                      "kernel_GBps": round(gbps, 1), "kernel_frac_of_peak": round(gbps / HBM_PEAK_GBS, 3),
+                      "bytes_per_launch": st["bytes"] / st["launches"],
+                      "per_call_upload_ms": round(1e3 * once, 1)}
+        log(f"bench: C2 20q CSR {name}: kernel {k_ms:.3f} ms, {gbps:.0f} GB/s")
+    csr = {"qubits": n20, "nnz": int(d.size), "index_bytes": 4,
+           "bytes_per_nnz": {"complex128": 20, "complex64": 12}, **rows,
+           "note": ("kernel bytes = nnz * (value + index) + (rows + 1) * index + state; "
+                    "per_call_upload_ms is plk_expval_csr_i32 with host arrays (PCIe upload included)")}
+    return {"stream_26q": stream, "hamiltonian_64_words_26q": ham_line, "csr_20q": csr}
+
+
+def copy_ceiling(capi, n=24, reps=10):
+    """Device-to-device copies of a complex128 state (hipMemcpyAsync): read +
+    written bytes over time, the practical HBM ceiling of a streaming pass."""
+    a, b = capi.State(n), capi.State(n)
+    b.copy_from(a)
+    capi.profiling_enable(True)
+    for _ in range(reps):
+        b.copy_from(a)
+    b.synchronize()
+    st = _collect(capi)["copy"]
+    return round(st["bytes"] / (st["total_ms"] * 1e-3) / 1e9, 1)
 
 
 def run_batched(capi, count, seq_count):
@@ -180,6 +351,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--c64-steps", type=int, default=3)
+    ap.add_argument("--c2", type=int, default=1, help="run the gate-throughput workload (0 skips it)")
     ap.add_argument("--batched", type=int, default=4096,
                     help="circuits of the batched workload (0 skips it)")
     ap.add_argument("--batched-seq", type=int, default=4096,
@@ -212,6 +384,8 @@ def main():
     value = ntp * args.steps / dt
     log(f"bench: complex128 {ms:.1f} ms/step, {value:.1f} params/s")
 
+    ceiling = copy_ceiling(capi)
+    log(f"bench: state copy {ceiling} GB/s")
     fams = [k for k in stats if k.startswith("adjoint_")]
     dom = max(fams, key=lambda k: stats[k]["total_ms"]) if fams else None
     seg = stats.get(dom, {"launches": 0, "total_ms": 0, "bytes": 0})
@@ -222,38 +396,45 @@ def main():
         achieved = avg_bytes / (avg_ms * 1e-3) / 1e9
         roof = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(dom),
-                "kernel": {"adjoint_tile": "tile_kernel (LDS tile pass, adjoint)",
+                "kernel": {"adjoint_rtile": "rtile_kernel (register-tile pass, adjoint)",
+                           "adjoint_tile": "tile_kernel (LDS tile pass, adjoint)",
                            "adjoint_segment": "seg_kernel (register segment, adjoint)"}.get(dom, dom),
                 "avg_launch_ms": round(avg_ms, 4), "avg_launch_bytes": avg_bytes,
-                "launches_per_step": seg["launches"] / args.steps}
+                "launches_per_step": seg["launches"] / args.steps,
+                "copy_ceiling_GBps": ceiling, "frac_of_copy_ceiling": round(achieved / ceiling, 4)}
     breakdown = {k: {"ms_per_step": round(v["total_ms"] / args.steps, 3),
                      "launches_per_step": v["launches"] / args.steps,
                      "GBps": round(v["bytes"] / max(v["total_ms"], 1e-9) / 1e6, 1)}
                  for k, v in stats.items()}
 
-    # Sweep-level accounting of the unfused algorithm: per trainable rotation
-    # 2 S (1 + n_obs) bytes, per CNOT S (1 + n_obs) (every state read and
-    # written once per gate). Fused passes move far fewer bytes; this figure
-    # states what the same sweep would cost one gate per pass.
+    # Sweep-level accounting: the unfused algorithm moves 2 S (1 + n_obs) bytes
+    # per trainable rotation and S (1 + n_obs) per CNOT (every state read and
+    # written once per gate). The fused passes move far fewer bytes;
+    # unfused_equivalent_GBps states the bandwidth a one-gate-per-pass sweep
+    # would need to match this sweep's time (it is not a measured bandwidth).
     S = (2 ** n) * 16
     n_rot = sum(1 for p in ops[1] if p)
     n_other = len(ops[0]) - n_rot
     unfused = (2 * n_rot + n_other) * S * (1 + len(obs))
     adj_ms = sum(v["total_ms"] for k, v in stats.items()
                  if k.startswith("adjoint_") or k.startswith("reduce")) / args.steps
-    passes = seg["launches"] / args.steps / max(1, -(-len(obs) // 4)) if seg["launches"] else None
+    fwd_ms = sum(v["total_ms"] for k, v in stats.items() if k.startswith("apply_")) / args.steps
+    chunks = max(1, -(-len(obs) // 3))
+    passes = seg["launches"] / args.steps / chunks if seg["launches"] else None
     extra = {"sweep": {"unfused_algorithmic_bytes": unfused, "adjoint_ms_per_step": round(adj_ms, 3),
-                       "effective_GBps": round(unfused / (adj_ms * 1e-3) / 1e9, 1),
-                       "effective_frac_of_peak": round(unfused / (adj_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 3),
-                       "ops_per_adjoint_pass": round(len(ops[0]) / passes, 1) if passes else None,
-                       "params_per_s_at_70pct_unfused": round(
-                           ntp / (unfused / (0.7 * HBM_PEAK_GBS * 1e9) + (ms - adj_ms) * 1e-3), 1)}}
+                       "forward_ms_per_step": round(fwd_ms, 3),
+                       "unfused_equivalent_GBps": round(unfused / (adj_ms * 1e-3) / 1e9, 1),
+                       "ops_per_adjoint_pass": round(len(ops[0]) / passes, 1) if passes else None}}
     if args.c64_steps > 0:
         log("bench: complex64 (FP64 reductions)")
         jac64, dt64, _, _ = run_gpu(capi, n, ops, obs, np.complex64, args.c64_steps, 1, profile=False)
         extra["complex64"] = {"params_per_s": round(ntp * args.c64_steps / dt64, 1),
                               "ms_per_step": round(1e3 * dt64 / args.c64_steps, 2),
                               "max_abs_diff_vs_complex128": float(np.max(np.abs(jac64 - jac)))}
+
+    if args.c2:
+        log("bench: gate-throughput workload (26-qubit stream, 64-word <H>, 20-qubit CSR)")
+        extra["c2"] = run_c2(capi)
 
     if args.batched > 0:
         log(f"bench: batched workload ({args.batched} circuits)")

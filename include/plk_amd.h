@@ -135,6 +135,14 @@ int plk_expval_matrix(const plk_state *sv, const double *matrix,
 int plk_expval_csr(const plk_state *sv, const double *data,
                    const int64_t *indices, const int64_t *indptr, size_t nnz,
                    size_t num_rows, double *out);
+/* plk_expval_csr with 32-bit indices and indptr, as the reference's
+ * serialiser sends them for complex64 (_serialize.py:136-154); the arrays
+ * reach the kernels without widening. Malformed matrices (decreasing indptr,
+ * an index outside [0, num_rows)) fail with PLK_ERR_INVALID, detected by the
+ * kernels. */
+int plk_expval_csr_i32(const plk_state *sv, const double *data,
+                       const int32_t *indices, const int32_t *indptr,
+                       size_t nnz, size_t num_rows, double *out);
 /* Re<psi|O|psi> of an observable tree (MeasuresKokkos.hpp:354-360). */
 int plk_expval_obs(const plk_state *sv, const plk_obs *ob, double *out);
 /* plk_expval_obs for an observable that is a sum of at most 4096 products of
@@ -174,6 +182,13 @@ int plk_obs_hamiltonian(const double *coeffs, const plk_obs *const *terms,
 int plk_obs_sparse(const double *data, const int64_t *indices,
                    const int64_t *indptr, size_t nnz, size_t num_rows,
                    const size_t *wires, size_t num_wires, plk_obs **out);
+/* SparseHamiltonianKokkos_C64 with the int32 indices of the complex64
+ * serialisation (_serialize.py:136-154). Every sparse observable keeps one
+ * device copy per device and state precision after its first use (values in
+ * the state's precision, 32-bit indices whenever they fit). */
+int plk_obs_sparse_i32(const double *data, const int32_t *indices,
+                       const int32_t *indptr, size_t nnz, size_t num_rows,
+                       const size_t *wires, size_t num_wires, plk_obs **out);
 int plk_obs_destroy(plk_obs *ob);
 /* Sorted wires of an observable (get_wires). *count receives the number of
  * wires; at most cap are written. */

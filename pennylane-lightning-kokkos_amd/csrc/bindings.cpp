@@ -167,10 +167,32 @@ std::vector<double> complex_doubles(
     return d;
 }
 
+// CSR entry points by index width.
+int csr_expval(plk_state *sv, const double *d, const int32_t *i, const int32_t *p, size_t nnz,
+               size_t rows, double *out) {
+    return plk_expval_csr_i32(sv, d, i, p, nnz, rows, out);
+}
+int csr_expval(plk_state *sv, const double *d, const int64_t *i, const int64_t *p, size_t nnz,
+               size_t rows, double *out) {
+    return plk_expval_csr(sv, d, i, p, nnz, rows, out);
+}
+int obs_sparse(const double *d, const int32_t *i, const int32_t *p, size_t nnz, size_t rows,
+               const size_t *w, size_t nw, plk_obs **out) {
+    return plk_obs_sparse_i32(d, i, p, nnz, rows, w, nw, out);
+}
+int obs_sparse(const double *d, const int64_t *i, const int64_t *p, size_t nnz, size_t rows,
+               const size_t *w, size_t nw, plk_obs **out) {
+    return plk_obs_sparse(d, i, p, nnz, rows, w, nw, out);
+}
+
 template <class PrecisionT> void bind_precision(py::module_ &m) {
     using SV = StateVector<PrecisionT>;
     using C = std::complex<PrecisionT>;
     using np_c = py::array_t<C, py::array::c_style | py::array::forcecast>;
+    // CSR index width per precision, as the reference's serialiser sends it
+    // (_serialize.py:136-154): int32 for complex64, int64 for complex128.
+    using IndexT = std::conditional_t<std::is_same_v<PrecisionT, float>, int32_t, int64_t>;
+    using np_idx = py::array_t<IndexT, py::array::c_style | py::array::forcecast>;
     using np_r = py::array_t<double, py::array::c_style | py::array::forcecast>;
     const std::string sfx = Traits<PrecisionT>::suffix;
 
@@ -308,12 +330,15 @@ template <class PrecisionT> void bind_precision(py::module_ &m) {
              },
              "Calculate the expectation value of the given observable.")
         .def("ExpectationValue",
-             [](SV &sv, const np_c &data, const std::vector<int64_t> &indices,
-                const std::vector<int64_t> &indptr) {
+             [](SV &sv, const np_c &data, const np_idx &indices, const np_idx &indptr) {
                  auto d = complex_doubles<PrecisionT>(data);
                  double out = 0;
-                 chk(plk_expval_csr(sv.h, d.data(), indices.data(), indptr.data(), indices.size(),
-                                    indptr.size() - 1, &out));
+                 if (indptr.size() == 0) {
+                     throw EngineError("CSR indptr is empty");
+                 }
+                 chk(csr_expval(sv.h, d.data(), indices.data(), indptr.data(),
+                                static_cast<size_t>(indices.size()),
+                                static_cast<size_t>(indptr.size()) - 1, &out));
                  return out;
              },
              "Calculate the expectation value of the given observable.")
@@ -480,16 +505,18 @@ template <class PrecisionT> void bind_precision(py::module_ &m) {
         using SH = SparseHamiltonian<PrecisionT>;
         py::class_<SH, std::shared_ptr<SH>, OB> c(
             m, ("SparseHamiltonianKokkos_" + sfx).c_str(), py::module_local());
-        c.def(py::init([](const np_c &data, const std::vector<int64_t> &indices,
-                          const std::vector<int64_t> &indptr, const std::vector<size_t> &wires) {
+        c.def(py::init([](const np_c &data, const np_idx &indices, const np_idx &indptr,
+                          const std::vector<size_t> &wires) {
             auto o = std::make_shared<SH>();
             auto d = complex_doubles<PrecisionT>(data);
-            if (d.size() / 2 != indices.size()) {
+            if (d.size() / 2 != static_cast<size_t>(indices.size())) {
                 throw EngineError("SparseHamiltonian: data and indices differ in length");
             }
             plk_obs *h = nullptr;
-            chk(plk_obs_sparse(d.data(), indices.data(), indptr.data(), indices.size(),
-                               indptr.empty() ? 0 : indptr.size() - 1, wires.data(), wires.size(), &h));
+            chk(obs_sparse(d.data(), indices.data(), indptr.data(),
+                           static_cast<size_t>(indices.size()),
+                           indptr.size() == 0 ? 0 : static_cast<size_t>(indptr.size()) - 1,
+                           wires.data(), wires.size(), &h));
             o->h = own(h);
             o->kind = "SparseHamiltonian";
             std::ostringstream rep;
@@ -498,12 +525,12 @@ template <class PrecisionT> void bind_precision(py::module_ &m) {
                 rep << "(" << d[2 * i] << "," << d[2 * i + 1] << ")";
             }
             rep << ",\n'indices' : ";
-            for (auto i : indices) {
-                rep << i;
+            for (py::ssize_t i = 0; i < indices.size(); i++) {
+                rep << indices.data()[i];
             }
             rep << ",\n'indptr' : ";
-            for (auto i : indptr) {
-                rep << i;
+            for (py::ssize_t i = 0; i < indptr.size(); i++) {
+                rep << indptr.data()[i];
             }
             rep << "\n}";
             o->repr = rep.str();

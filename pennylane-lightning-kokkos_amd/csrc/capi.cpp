@@ -34,8 +34,34 @@ struct ObsNode {
     std::vector<std::shared_ptr<const ObsNode>> terms;
     std::vector<double> coeffs;
     std::vector<double> data; // CSR values, interleaved complex
-    std::vector<int64_t> indices, indptr;
+    std::vector<int64_t> indices, indptr;       // int64 matrices
+    std::vector<int32_t> indices32, indptr32;   // int32 matrices (complex64 serialisation)
+    bool idx32 = false;
     size_t rows = 0;
+    // Device copies of the CSR matrix per (device, state dtype), made on first
+    // use and kept while the observable lives (entry points hold the engine
+    // lock, so the cache needs no lock of its own).
+    mutable std::map<std::pair<int, int>, std::unique_ptr<CsrDevice>> csr_dev;
+
+    CsrView csr_view() const {
+        CsrView v;
+        v.data = data.data();
+        v.idx32 = idx32;
+        v.indices = idx32 ? static_cast<const void *>(indices32.data()) : indices.data();
+        v.indptr = idx32 ? static_cast<const void *>(indptr32.data()) : indptr.data();
+        v.nnz = data.size() / 2;
+        v.rows = rows;
+        return v;
+    }
+    const CsrDevice &csr_on(Device &dev, const State &s) const {
+        auto &slot = csr_dev[{dev.id, s.dtype}];
+        if (!slot) {
+            const CsrView v = csr_view();
+            csr_check_shape(v, s.len);
+            slot = csr_upload(dev, v, s.dtype);
+        }
+        return *slot;
+    }
 };
 
 std::vector<size_t> obs_wires(const ObsNode &o) {
@@ -1365,10 +1391,21 @@ void apply_generator_dev(Device &dev, const State &s, const CompiledGen &g) {
 // ---------------------------------------------------------------------------
 void apply_obs_general(Device &dev, const State &s, const ObsNode &o);
 
+void check_sparse_wires(const ObsNode &o, const State &s) {
+    if (o.wires.size() != static_cast<size_t>(s.n)) {
+        raise(PLK_ERR_INVALID, "SparseH wire count does not match state-vector size");
+    }
+}
+
 void apply_obs_into(Device &dev, const State &out, const State &in, const ObsNode &o) {
     std::vector<PauliTerm> terms;
     if (to_pauli(o, in.n, terms)) {
         launch_pauli_apply(dev, out, in, terms);
+        return;
+    }
+    if (o.kind == ObsNode::Sparse) {
+        check_sparse_wires(o, in);
+        csr_spmv(dev, out, in, o.csr_on(dev, in));
         return;
     }
     launch_copy(dev, out, in);
@@ -1406,12 +1443,9 @@ void apply_obs_general(Device &dev, const State &s, const ObsNode &o) {
         return;
     }
     case ObsNode::Sparse: {
-        if (o.wires.size() != static_cast<size_t>(s.n)) {
-            raise(PLK_ERR_INVALID, "SparseH wire count does not match state-vector size");
-        }
+        check_sparse_wires(o, s);
         TmpState tmp(s.n, s.dtype, s.device);
-        csr_spmv(dev, tmp.s, s, o.data.data(), o.indices.data(), o.indptr.data(),
-                 o.indices.size(), o.rows);
+        csr_spmv(dev, tmp.s, s, o.csr_on(dev, s));
         launch_copy(dev, s, tmp.s);
         PLK_HIP(hipStreamSynchronize(dev.stream));
         return;
@@ -1423,6 +1457,11 @@ double expval_obs(Device &dev, const State &s, const ObsNode &o) {
     std::vector<PauliTerm> terms;
     if (to_pauli(o, s.n, terms)) {
         return pauli_expval(dev, s, terms);
+    }
+    if (o.kind == ObsNode::Sparse) {
+        // sum_r Re(conj(psi_r) (A psi)_r) without materialising A psi.
+        check_sparse_wires(o, s);
+        return csr_expval(dev, s, o.csr_on(dev, s));
     }
     TmpState tmp(s.n, s.dtype, s.device);
     launch_copy(dev, tmp.s, s);
@@ -2280,7 +2319,32 @@ int plk_expval_csr(const plk_state *sv, const double *data, const int64_t *indic
                    const int64_t *indptr, size_t nnz, size_t num_rows, double *out) {
     PLK_TRY({
         const State &s = S(sv);
-        *out = csr_expval(dev_of(s), s, data, indices, indptr, nnz, num_rows);
+        CsrView v;
+        v.data = data;
+        v.indices = indices;
+        v.indptr = indptr;
+        v.idx32 = false;
+        v.nnz = nnz;
+        v.rows = num_rows;
+        *out = csr_expval_once(dev_of(s), s, v);
+    })
+}
+
+int plk_expval_csr_i32(const plk_state *sv, const double *data, const int32_t *indices,
+                       const int32_t *indptr, size_t nnz, size_t num_rows, double *out) {
+    PLK_TRY({
+        const State &s = S(sv);
+        if (num_rows > size_t{1} << 31 || nnz > static_cast<size_t>(INT32_MAX)) {
+            raise(PLK_ERR_INVALID, "CSR matrix too large for 32-bit indices");
+        }
+        CsrView v;
+        v.data = data;
+        v.indices = indices;
+        v.indptr = indptr;
+        v.idx32 = true;
+        v.nnz = nnz;
+        v.rows = num_rows;
+        *out = csr_expval_once(dev_of(s), s, v);
     })
 }
 
@@ -2384,6 +2448,25 @@ int plk_obs_sparse(const double *data, const int64_t *indices, const int64_t *in
         n->data.assign(data, data + 2 * nnz);
         n->indices.assign(indices, indices + nnz);
         n->indptr.assign(indptr, indptr + num_rows + 1);
+        n->rows = num_rows;
+        n->wires = to_vec(wires, num_wires);
+        *out = new plk_obs{n};
+    })
+}
+
+int plk_obs_sparse_i32(const double *data, const int32_t *indices, const int32_t *indptr,
+                       size_t nnz, size_t num_rows, const size_t *wires, size_t num_wires,
+                       plk_obs **out) {
+    PLK_TRY({
+        if (num_rows > size_t{1} << 31 || nnz > static_cast<size_t>(INT32_MAX)) {
+            raise(PLK_ERR_INVALID, "CSR matrix too large for 32-bit indices");
+        }
+        auto n = std::make_shared<ObsNode>();
+        n->kind = ObsNode::Sparse;
+        n->idx32 = true;
+        n->data.assign(data, data + 2 * nnz);
+        n->indices32.assign(indices, indices + nnz);
+        n->indptr32.assign(indptr, indptr + num_rows + 1);
         n->rows = num_rows;
         n->wires = to_vec(wires, num_wires);
         *out = new plk_obs{n};

@@ -514,6 +514,7 @@ void launch_copy(Device &dev, const State &dst, const State &src) {
     if (dst.len != src.len || dst.dtype != src.dtype) {
         raise(PLK_ERR_INVALID, "state copy between different shapes");
     }
+    ProfScope prof("copy", 2.0 * static_cast<double>(src.bytes()), dev.stream);
     PLK_HIP(hipMemcpyAsync(dst.d, src.d, src.bytes(), hipMemcpyDeviceToDevice, dev.stream));
 }
 

@@ -196,45 +196,120 @@ __global__ void __launch_bounds__(kBlock)
     }
 }
 
-template <typename T>
+// CSR kernels: one wave works on R = 64 / G consecutive rows at a time, G
+// lanes per row striding over the row's entries, so a wave's loads of the
+// values and indices are contiguous runs (streamed with non-temporal loads:
+// they are read once), while the gathers psi[col] stay cacheable. Rows are
+// dealt to waves round-robin. Every row range and column index is checked:
+// a malformed matrix is skipped and counted in the block's flag slot
+// (partials[grid + block]) instead of being read out of bounds.
+template <typename V> struct Vec2;
+template <> struct Vec2<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <> struct Vec2<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+
+template <typename V> __device__ __forceinline__ Cx<V> ld_stream(const Cx<V> *p, uint64_t k) {
+    using W = typename Vec2<V>::type;
+    const W w = __builtin_nontemporal_load(reinterpret_cast<const W *>(p) + k);
+    return {w.x, w.y};
+}
+
+template <typename I> __device__ __forceinline__ int64_t ld_index(const I *p, uint64_t k) {
+    return static_cast<int64_t>(__builtin_nontemporal_load(p + k));
+}
+
+template <typename T, typename V, typename I, int G>
 __global__ void __launch_bounds__(kBlock)
-    csr_expval_kernel(const Cx<T> *__restrict__ psi, const double *__restrict__ data,
-                      const int64_t *__restrict__ indices, const int64_t *__restrict__ indptr,
-                      uint64_t rows, double *__restrict__ partials) {
-    double acc[1] = {0.0};
-    for (uint64_t r = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; r < rows;
-         r += uint64_t{gridDim.x} * blockDim.x) {
-        double sr = 0.0, si = 0.0;
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; k++) {
-            const Cx<T> b = ld(psi, static_cast<uint64_t>(indices[k]));
-            const double dr = data[2 * k], di = data[2 * k + 1];
-            sr += dr * b.r - di * b.i;
-            si += dr * b.i + di * b.r;
+    csr_expval_kernel(const Cx<T> *__restrict__ psi, uint64_t len,
+                      const Cx<V> *__restrict__ data, const I *__restrict__ indices,
+                      const I *__restrict__ indptr, uint64_t rows, uint64_t nnz,
+                      double *__restrict__ partials) {
+    constexpr int R = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & (G - 1);
+    const uint64_t wave = uint64_t{blockIdx.x} * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint64_t nwaves = uint64_t{gridDim.x} * (kBlock / 64);
+    double acc[2] = {0.0, 0.0};
+    for (uint64_t r = wave * R + lane / G; r < rows; r += nwaves * R) {
+        const int64_t start = indptr[r], end = indptr[r + 1];
+        if (start < 0 || end < start || static_cast<uint64_t>(end) > nnz) {
+            acc[1] = 1.0;
+            continue;
         }
-        const Cx<T> a = ld(psi, r);
-        acc[0] += static_cast<double>(a.r) * sr + static_cast<double>(a.i) * si;
+        double sr = 0.0, si = 0.0;
+        for (int64_t k = start + sub; k < end; k += G) {
+            const int64_t c = ld_index(indices, static_cast<uint64_t>(k));
+            if (c < 0 || static_cast<uint64_t>(c) >= len) {
+                acc[1] = 1.0;
+                continue;
+            }
+            const Cx<V> a = ld_stream(data, static_cast<uint64_t>(k));
+            const Cx<T> b = ld(psi, static_cast<uint64_t>(c));
+            const double ar = a.r, ai = a.i, br = b.r, bi = b.i;
+            sr = fma(ar, br, fma(-ai, bi, sr));
+            si = fma(ar, bi, fma(ai, br, si));
+        }
+        // Re(conj(psi_r) * partial row sum): summed over the row's lanes later.
+        const Cx<T> p = ld(psi, r);
+        acc[0] = fma(static_cast<double>(p.r), sr, fma(static_cast<double>(p.i), si, acc[0]));
     }
-    block_sum<1>(acc);
+    block_sum<2>(acc);
     if (threadIdx.x == 0) {
         partials[blockIdx.x] = acc[0];
+        partials[gridDim.x + blockIdx.x] = acc[1];
     }
 }
 
-template <typename T>
+template <typename T, typename V, typename I, int G>
 __global__ void __launch_bounds__(kBlock)
-    csr_spmv_kernel(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ psi,
-                    const double *__restrict__ data, const int64_t *__restrict__ indices,
-                    const int64_t *__restrict__ indptr, uint64_t rows) {
-    for (uint64_t r = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; r < rows;
-         r += uint64_t{gridDim.x} * blockDim.x) {
+    csr_spmv_kernel(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ psi, uint64_t len,
+                    const Cx<V> *__restrict__ data, const I *__restrict__ indices,
+                    const I *__restrict__ indptr, uint64_t rows, uint64_t nnz,
+                    double *__restrict__ partials) {
+    constexpr int R = 64 / G;
+    const int lane = threadIdx.x & 63;
+    const int sub = lane & (G - 1);
+    const uint64_t wave = uint64_t{blockIdx.x} * (kBlock / 64) + (threadIdx.x >> 6);
+    const uint64_t nwaves = uint64_t{gridDim.x} * (kBlock / 64);
+    double bad[1] = {0.0};
+    // Wave-uniform loop: every lane takes part in the row-sum shuffles.
+    for (uint64_t r0 = wave * R; r0 < rows; r0 += nwaves * R) {
+        const uint64_t r = r0 + lane / G;
         double sr = 0.0, si = 0.0;
-        for (int64_t k = indptr[r]; k < indptr[r + 1]; k++) {
-            const Cx<T> b = ld(psi, static_cast<uint64_t>(indices[k]));
-            const double dr = data[2 * k], di = data[2 * k + 1];
-            sr += dr * b.r - di * b.i;
-            si += dr * b.i + di * b.r;
+        if (r < rows) {
+            const int64_t start = indptr[r], end = indptr[r + 1];
+            if (start < 0 || end < start || static_cast<uint64_t>(end) > nnz) {
+                bad[0] = 1.0;
+            } else {
+                for (int64_t k = start + sub; k < end; k += G) {
+                    const int64_t c = ld_index(indices, static_cast<uint64_t>(k));
+                    if (c < 0 || static_cast<uint64_t>(c) >= len) {
+                        bad[0] = 1.0;
+                        continue;
+                    }
+                    const Cx<V> a = ld_stream(data, static_cast<uint64_t>(k));
+                    const Cx<T> b = ld(psi, static_cast<uint64_t>(c));
+                    const double ar = a.r, ai = a.i, br = b.r, bi = b.i;
+                    sr = fma(ar, br, fma(-ai, bi, sr));
+                    si = fma(ar, bi, fma(ai, br, si));
+                }
+            }
         }
-        st(out, r, Cx<T>{static_cast<T>(sr), static_cast<T>(si)});
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1) {
+            sr += __shfl_xor(sr, off, 64);
+            si += __shfl_xor(si, off, 64);
+        }
+        if (sub == 0 && r < rows) {
+            st(out, r, Cx<T>{static_cast<T>(sr), static_cast<T>(si)});
+        }
+    }
+    block_sum<1>(bad);
+    if (threadIdx.x == 0) {
+        partials[blockIdx.x] = bad[0];
     }
 }
 
@@ -494,104 +569,219 @@ void launch_pauli_apply(Device &dev, const State &out, const State &in,
     free_tables(dev, t.blob);
 }
 
-namespace {
+CsrDevice::~CsrDevice() {
+    if (blob != nullptr) {
+        // Destructors do not throw: errors of the teardown are ignored.
+        (void)hipSetDevice(device);
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(blob);
+    }
+}
 
-struct CsrDev {
-    void *blob = nullptr;
-    const double *data = nullptr;
-    const int64_t *indices = nullptr;
-    const int64_t *indptr = nullptr;
-};
-
-CsrDev upload_csr(const State &s, const double *data, const int64_t *indices,
-                  const int64_t *indptr, size_t nnz, size_t rows) {
-    if (rows != s.len) {
-        raise(PLK_ERR_INVALID, "CSR matrix has " + std::to_string(rows) +
-                                   " rows but the state has " + std::to_string(s.len) +
+void csr_check_shape(const CsrView &m, size_t len) {
+    if (m.rows != len) {
+        raise(PLK_ERR_INVALID, "CSR matrix has " + std::to_string(m.rows) +
+                                   " rows but the state has " + std::to_string(len) +
                                    " amplitudes");
     }
-    if (static_cast<size_t>(indptr[rows]) != nnz || indptr[0] != 0) {
+    const int64_t first = m.idx32 ? static_cast<const int32_t *>(m.indptr)[0]
+                                  : static_cast<const int64_t *>(m.indptr)[0];
+    const int64_t last = m.idx32 ? static_cast<const int32_t *>(m.indptr)[m.rows]
+                                 : static_cast<const int64_t *>(m.indptr)[m.rows];
+    if (first != 0 || last < 0 || static_cast<size_t>(last) != m.nnz) {
         raise(PLK_ERR_INVALID, "CSR indptr does not match the number of non-zeros");
     }
-    for (size_t k = 0; k < nnz; k++) {
-        if (indices[k] < 0 || static_cast<size_t>(indices[k]) >= s.len) {
-            raise(PLK_ERR_INVALID, "CSR column index out of range");
+}
+
+namespace {
+
+// Lanes per row: the power of two nearest above the average row length,
+// from 4 to 64.
+int csr_lanes(size_t nnz, size_t rows) {
+    const double avg = rows ? static_cast<double>(nnz) / static_cast<double>(rows) : 0.0;
+    return avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 64);
+}
+
+unsigned csr_grid(const Device &dev, size_t rows, int G) {
+    const uint64_t rows_per_block = static_cast<uint64_t>(64 / G) * (kBlock / 64);
+    const uint64_t want = (rows + rows_per_block - 1) / rows_per_block;
+    const uint64_t cap = static_cast<uint64_t>(dev.num_cus) * 8;
+    return static_cast<unsigned>(std::max<uint64_t>(1, std::min(want, cap)));
+}
+
+// Raw device views of one CSR launch.
+struct CsrArgs {
+    const void *data, *indices, *indptr;
+    bool idx32, vdouble;
+    size_t nnz, rows;
+};
+
+template <typename T, typename V, typename I>
+void csr_launch_t(Device &dev, const CsrArgs &m, const State &in, Cx<T> *out, unsigned grid,
+                  int G, double *partials) {
+    const auto *psi = static_cast<const Cx<T> *>(in.d);
+    const auto *d = static_cast<const Cx<V> *>(m.data);
+    const auto *ix = static_cast<const I *>(m.indices);
+    const auto *ip = static_cast<const I *>(m.indptr);
+    const uint64_t len = in.len, rows = m.rows, nnz = m.nnz;
+#define PLK_CSR_LAUNCH(GG)                                                                        \
+    if (out == nullptr) {                                                                         \
+        hipLaunchKernelGGL((csr_expval_kernel<T, V, I, GG>), dim3(grid), dim3(kBlock), 0,       \
+                           dev.stream, psi, len, d, ix, ip, rows, nnz, partials);                 \
+    } else {                                                                                      \
+        hipLaunchKernelGGL((csr_spmv_kernel<T, V, I, GG>), dim3(grid), dim3(kBlock), 0,         \
+                           dev.stream, out, psi, len, d, ix, ip, rows, nnz, partials);            \
+    }
+    if (G == 4) {
+        PLK_CSR_LAUNCH(4)
+    } else if (G == 16) {
+        PLK_CSR_LAUNCH(16)
+    } else {
+        PLK_CSR_LAUNCH(64)
+    }
+#undef PLK_CSR_LAUNCH
+    PLK_HIP(hipGetLastError());
+}
+
+// One CSR kernel over `in`; out == nullptr computes the expectation value.
+// Returns {sum, number of blocks that met a malformed row or index}.
+std::pair<double, double> csr_run(Device &dev, const CsrArgs &m, const State &in,
+                                  const State *out) {
+    const int G = csr_lanes(m.nnz, m.rows);
+    const unsigned grid = csr_grid(dev, m.rows, G);
+    ensure_red(dev, static_cast<size_t>(grid) * 2, 2);
+    {
+        const double vb = m.vdouble ? 16.0 : 8.0;
+        const double ib = m.idx32 ? 4.0 : 8.0;
+        const double bytes = static_cast<double>(m.nnz) * (vb + ib) +
+                             ib * static_cast<double>(m.rows + 1) +
+                             (out ? 2.0 : 1.0) * static_cast<double>(in.bytes());
+        ProfScope prof(out ? "csr_spmv" : "csr_expval", bytes, dev.stream);
+        if (in.dtype == PLK_C64) {
+            auto *o = out ? static_cast<Cx<float> *>(out->d) : nullptr;
+            if (m.vdouble) {
+                if (m.idx32) {
+                    csr_launch_t<float, double, int32_t>(dev, m, in, o, grid, G, dev.red);
+                } else {
+                    csr_launch_t<float, double, int64_t>(dev, m, in, o, grid, G, dev.red);
+                }
+            } else if (m.idx32) {
+                csr_launch_t<float, float, int32_t>(dev, m, in, o, grid, G, dev.red);
+            } else {
+                csr_launch_t<float, float, int64_t>(dev, m, in, o, grid, G, dev.red);
+            }
+        } else {
+            if (!m.vdouble) {
+                raise(PLK_ERR_INVALID, "complex128 states need FP64 CSR values");
+            }
+            auto *o = out ? static_cast<Cx<double> *>(out->d) : nullptr;
+            if (m.idx32) {
+                csr_launch_t<double, double, int32_t>(dev, m, in, o, grid, G, dev.red);
+            } else {
+                csr_launch_t<double, double, int64_t>(dev, m, in, o, grid, G, dev.red);
+            }
         }
     }
-    for (size_t r = 0; r < rows; r++) {
-        if (indptr[r + 1] < indptr[r]) {
-            raise(PLK_ERR_INVALID, "CSR indptr must be non-decreasing");
-        }
+    double res[2] = {0.0, 0.0};
+    if (out == nullptr) {
+        finish_sums(dev, static_cast<int>(grid), 2, res);
+    } else {
+        // The flags sit where the expectation kernel keeps its sums.
+        finish_sums(dev, static_cast<int>(grid), 1, res + 1);
     }
-    const size_t db = nnz * 2 * sizeof(double);
-    const size_t ib = nnz * sizeof(int64_t);
-    const size_t pb = (rows + 1) * sizeof(int64_t);
-    const size_t o1 = (db + 255) / 256 * 256;
-    const size_t o2 = o1 + (ib + 255) / 256 * 256;
-    CsrDev c;
-    PLK_HIP(hipMalloc(&c.blob, o2 + pb + 16));
-    auto *b = static_cast<unsigned char *>(c.blob);
-    if (nnz) {
-        PLK_HIP(hipMemcpy(b, data, db, hipMemcpyHostToDevice));
-        PLK_HIP(hipMemcpy(b + o1, indices, ib, hipMemcpyHostToDevice));
+    if (res[1] != 0.0) {
+        raise(PLK_ERR_INVALID,
+              "CSR matrix has a row range or column index outside the matrix "
+              "(indptr must be non-decreasing, 0 <= indices < rows)");
     }
-    PLK_HIP(hipMemcpy(b + o2, indptr, pb, hipMemcpyHostToDevice));
-    c.data = reinterpret_cast<const double *>(b);
-    c.indices = reinterpret_cast<const int64_t *>(b + o1);
-    c.indptr = reinterpret_cast<const int64_t *>(b + o2);
-    return c;
+    return {res[0], res[1]};
+}
+
+CsrArgs args_of(const CsrDevice &m) {
+    return {m.data, m.indices, m.indptr, m.idx32, m.dtype == PLK_C128, m.nnz, m.rows};
+}
+
+template <typename Src, typename Dst> void narrow_copy(const Src *src, size_t n, Dst *dst) {
+    for (size_t k = 0; k < n; k++) {
+        dst[k] = static_cast<Dst>(src[k]);
+    }
 }
 
 } // namespace
 
-double csr_expval(Device &dev, const State &s, const double *data, const int64_t *indices,
-                  const int64_t *indptr, size_t nnz, size_t rows) {
-    CsrDev c = upload_csr(s, data, indices, indptr, nnz, rows);
-    const unsigned grid = grid_for(dev, rows);
-    ensure_red(dev, grid, 1);
-    {
-        ProfScope prof("csr_expval",
-                       static_cast<double>(nnz) * 24.0 + 8.0 * (rows + 1) + 2.0 * s.bytes(),
-                       dev.stream);
-        if (s.dtype == PLK_C64) {
-            hipLaunchKernelGGL((csr_expval_kernel<float>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<const Cx<float> *>(s.d), c.data,
-                               c.indices, c.indptr, static_cast<uint64_t>(rows), dev.red);
-        } else {
-            hipLaunchKernelGGL((csr_expval_kernel<double>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<const Cx<double> *>(s.d), c.data,
-                               c.indices, c.indptr, static_cast<uint64_t>(rows), dev.red);
-        }
-        PLK_HIP(hipGetLastError());
+std::unique_ptr<CsrDevice> csr_upload(Device &dev, const CsrView &m, int dtype) {
+    auto c = std::make_unique<CsrDevice>();
+    c->device = dev.id;
+    c->stream = dev.stream;
+    c->dtype = dtype;
+    c->nnz = m.nnz;
+    c->rows = m.rows;
+    c->idx32 = m.rows <= (size_t{1} << 31) && m.nnz <= static_cast<size_t>(INT32_MAX);
+    const size_t vb = dtype == PLK_C64 ? 8 : 16;
+    const size_t ib = c->idx32 ? 4 : 8;
+    const size_t o1 = (m.nnz * vb + 255) / 256 * 256;
+    const size_t o2 = o1 + (m.nnz * ib + 255) / 256 * 256;
+    const size_t total = o2 + (m.rows + 1) * ib + 16;
+    // Host image in the device layout, then one copy.
+    std::vector<unsigned char> img(total);
+    if (dtype == PLK_C64) {
+        narrow_copy(m.data, 2 * m.nnz, reinterpret_cast<float *>(img.data()));
+    } else if (m.nnz) {
+        std::memcpy(img.data(), m.data, m.nnz * 16);
     }
-    double out = 0.0;
-    finish_sums(dev, static_cast<int>(grid), 1, &out);
-    free_tables(dev, c.blob);
-    return out;
+    auto put_index = [&](const void *src, size_t n, size_t off) {
+        if (c->idx32 && m.idx32) {
+            std::memcpy(img.data() + off, src, n * 4);
+        } else if (c->idx32) {
+            narrow_copy(static_cast<const int64_t *>(src), n,
+                        reinterpret_cast<int32_t *>(img.data() + off));
+        } else if (m.idx32) {
+            narrow_copy(static_cast<const int32_t *>(src), n,
+                        reinterpret_cast<int64_t *>(img.data() + off));
+        } else {
+            std::memcpy(img.data() + off, src, n * 8);
+        }
+    };
+    put_index(m.indices, m.nnz, o1);
+    put_index(m.indptr, m.rows + 1, o2);
+    c->blob = dev_malloc(dev.id, total, "CSR matrix");
+    auto *b = static_cast<unsigned char *>(c->blob);
+    PLK_HIP(hipMemcpy(b, img.data(), total, hipMemcpyHostToDevice));
+    c->data = b;
+    c->indices = b + o1;
+    c->indptr = b + o2;
+    return c;
 }
 
-void csr_spmv(Device &dev, const State &out, const State &in, const double *data,
-              const int64_t *indices, const int64_t *indptr, size_t nnz, size_t rows) {
-    CsrDev c = upload_csr(in, data, indices, indptr, nnz, rows);
-    const unsigned grid = grid_for(dev, rows);
-    {
-        ProfScope prof("csr_spmv",
-                       static_cast<double>(nnz) * 24.0 + 8.0 * (rows + 1) + 2.0 * in.bytes(),
-                       dev.stream);
-        if (in.dtype == PLK_C64) {
-            hipLaunchKernelGGL((csr_spmv_kernel<float>), dim3(grid), dim3(kBlock), 0, dev.stream,
-                               static_cast<Cx<float> *>(out.d),
-                               static_cast<const Cx<float> *>(in.d), c.data, c.indices,
-                               c.indptr, static_cast<uint64_t>(rows));
-        } else {
-            hipLaunchKernelGGL((csr_spmv_kernel<double>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<Cx<double> *>(out.d),
-                               static_cast<const Cx<double> *>(in.d), c.data, c.indices,
-                               c.indptr, static_cast<uint64_t>(rows));
-        }
-        PLK_HIP(hipGetLastError());
+double csr_expval(Device &dev, const State &s, const CsrDevice &m) {
+    if (m.rows != s.len) {
+        raise(PLK_ERR_INVALID, "CSR matrix does not match the state's length");
     }
-    free_tables(dev, c.blob);
+    return csr_run(dev, args_of(m), s, nullptr).first;
+}
+
+double csr_expval_once(Device &dev, const State &s, const CsrView &m) {
+    csr_check_shape(m, s.len);
+    const size_t ib = m.idx32 ? 4 : 8;
+    const size_t o1 = (m.nnz * 16 + 255) / 256 * 256;
+    const size_t o2 = o1 + (m.nnz * ib + 255) / 256 * 256;
+    const size_t total = o2 + (m.rows + 1) * ib + 16;
+    auto *b = static_cast<unsigned char *>(ensure_work(dev, total));
+    if (m.nnz) {
+        PLK_HIP(hipMemcpyAsync(b, m.data, m.nnz * 16, hipMemcpyHostToDevice, dev.stream));
+        PLK_HIP(hipMemcpyAsync(b + o1, m.indices, m.nnz * ib, hipMemcpyHostToDevice, dev.stream));
+    }
+    PLK_HIP(hipMemcpyAsync(b + o2, m.indptr, (m.rows + 1) * ib, hipMemcpyHostToDevice,
+                           dev.stream));
+    const CsrArgs a{b, b + o1, b + o2, m.idx32, true, m.nnz, m.rows};
+    return csr_run(dev, a, s, nullptr).first;
+}
+
+void csr_spmv(Device &dev, const State &out, const State &in, const CsrDevice &m) {
+    if (m.rows != in.len || out.len != in.len || out.dtype != in.dtype) {
+        raise(PLK_ERR_INVALID, "CSR matrix does not match the state's length");
+    }
+    csr_run(dev, args_of(m), in, &out);
 }
 
 } // namespace plk

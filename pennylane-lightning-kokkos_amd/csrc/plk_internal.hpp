@@ -397,13 +397,44 @@ constexpr size_t kNoAllocMaxPauliWords = 4096;
 // PLK_ERR_INVALID for more than kNoAllocMaxPauliWords terms.
 double pauli_expval_noalloc(Device &dev, const State &s, const std::vector<PauliTerm> &terms);
 
-// CSR matrix (device copies made per call).
-double csr_expval(Device &dev, const State &s, const double *data,
-                  const int64_t *indices, const int64_t *indptr, size_t nnz,
-                  size_t rows);
-void csr_spmv(Device &dev, const State &out, const State &in, const double *data,
-              const int64_t *indices, const int64_t *indptr, size_t nnz,
-              size_t rows);
+// A CSR matrix as handed over at the boundary: borrowed host arrays, values
+// interleaved FP64 complex, indices and indptr both int32 or both int64.
+struct CsrView {
+    const double *data = nullptr;
+    const void *indices = nullptr;
+    const void *indptr = nullptr;
+    bool idx32 = false;
+    size_t nnz = 0;
+    size_t rows = 0;
+};
+// A device copy of a CSR matrix kept for the life of its observable: values
+// in the state's precision, 32-bit indices whenever rows and nnz fit.
+struct CsrDevice {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    int dtype = 0;
+    bool idx32 = true;
+    size_t nnz = 0, rows = 0;
+    void *blob = nullptr;
+    const void *data = nullptr, *indices = nullptr, *indptr = nullptr;
+    CsrDevice() = default;
+    CsrDevice(const CsrDevice &) = delete;
+    CsrDevice &operator=(const CsrDevice &) = delete;
+    ~CsrDevice();
+};
+// O(1) checks of the shape (rows == state length, indptr[0] == 0,
+// indptr[rows] == nnz); the kernels check every index and row range and
+// report violations, so no O(nnz) host pass is made.
+void csr_check_shape(const CsrView &m, size_t len);
+std::unique_ptr<CsrDevice> csr_upload(Device &dev, const CsrView &m, int dtype);
+// sum_r Re(conj(psi_r) sum_j A_rj psi_cj) (MeasuresKokkos.hpp:132-157).
+double csr_expval(Device &dev, const State &s, const CsrDevice &m);
+// The same for arrays handed over per call: copied into the device
+// workspace (no allocation once it has grown), FP64 values.
+double csr_expval_once(Device &dev, const State &s, const CsrView &m);
+// out = A in (LinearAlgebraKokkos.hpp:68-148); synchronizes to report
+// malformed matrices.
+void csr_spmv(Device &dev, const State &out, const State &in, const CsrDevice &m);
 
 // Probabilities and sampling (kernels_sample.hip). probs: 2^m FP64 values,
 // bin bit (m-1-k) taken from global bit gbits[k]. sample: shots rows of n

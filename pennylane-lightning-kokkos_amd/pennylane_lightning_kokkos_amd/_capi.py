@@ -23,6 +23,7 @@ PLK_C128 = 128
 _c_double_p = ctypes.POINTER(ctypes.c_double)
 _c_size_p = ctypes.POINTER(ctypes.c_size_t)
 _c_int64_p = ctypes.POINTER(ctypes.c_int64)
+_c_int32_p = ctypes.POINTER(ctypes.c_int32)
 _vp = ctypes.c_void_p
 
 # name -> (restype, argtypes) for every entry point of include/plk_amd.h
@@ -53,6 +54,8 @@ SIGNATURES = {
     "plk_expval_matrix": (ctypes.c_int, [_vp, _c_double_p, _c_size_p, ctypes.c_size_t, _c_double_p]),
     "plk_expval_csr": (ctypes.c_int, [_vp, _c_double_p, _c_int64_p, _c_int64_p, ctypes.c_size_t,
                                       ctypes.c_size_t, _c_double_p]),
+    "plk_expval_csr_i32": (ctypes.c_int, [_vp, _c_double_p, _c_int32_p, _c_int32_p, ctypes.c_size_t,
+                                          ctypes.c_size_t, _c_double_p]),
     "plk_expval_obs": (ctypes.c_int, [_vp, _vp, _c_double_p]),
     "plk_expval_obs_noalloc": (ctypes.c_int, [_vp, _vp, _c_double_p]),
     "plk_inner_product": (ctypes.c_int, [_vp, _vp, _c_double_p, _c_double_p]),
@@ -63,6 +66,8 @@ SIGNATURES = {
                                            ctypes.POINTER(_vp)]),
     "plk_obs_sparse": (ctypes.c_int, [_c_double_p, _c_int64_p, _c_int64_p, ctypes.c_size_t,
                                       ctypes.c_size_t, _c_size_p, ctypes.c_size_t, ctypes.POINTER(_vp)]),
+    "plk_obs_sparse_i32": (ctypes.c_int, [_c_double_p, _c_int32_p, _c_int32_p, ctypes.c_size_t,
+                                          ctypes.c_size_t, _c_size_p, ctypes.c_size_t, ctypes.POINTER(_vp)]),
     "plk_obs_destroy": (ctypes.c_int, [_vp]),
     "plk_obs_wires": (ctypes.c_int, [_vp, _c_size_p, ctypes.c_size_t, _c_size_p]),
     "plk_obs_apply": (ctypes.c_int, [_vp, _vp]),
@@ -113,6 +118,16 @@ def load(path: Optional[str] = None):
 def check(rc: int):
     if rc != PLK_OK:
         raise PLException(load().plk_last_error().decode())
+
+
+def _csr_indices(indices, indptr):
+    """(indices, indptr, is_int32): int32 arrays stay int32, anything else
+    becomes int64."""
+    i = np.asarray(indices).reshape(-1)
+    p = np.asarray(indptr).reshape(-1)
+    i32 = i.dtype == np.int32 and p.dtype == np.int32
+    t = np.int32 if i32 else np.int64
+    return np.ascontiguousarray(i, dtype=t), np.ascontiguousarray(p, dtype=t), i32
 
 
 def _sizes(x) -> np.ndarray:
@@ -246,12 +261,19 @@ class State:
         return out.value
 
     def expval_csr(self, data, indices, indptr) -> float:
+        """CSR expectation value; int32 index arrays (scipy's default, the
+        complex64 serialisation) go through plk_expval_csr_i32 unwidened."""
         d = _cplx_doubles(data)
-        i = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
-        p = np.ascontiguousarray(np.asarray(indptr, dtype=np.int64).reshape(-1))
+        i, p, i32 = _csr_indices(indices, indptr)
         out = ctypes.c_double()
-        check(load().plk_expval_csr(self.h, _dp(d), i.ctypes.data_as(_c_int64_p),
-                                    p.ctypes.data_as(_c_int64_p), i.size, p.size - 1, ctypes.byref(out)))
+        if i32:
+            check(load().plk_expval_csr_i32(self.h, _dp(d), i.ctypes.data_as(_c_int32_p),
+                                            p.ctypes.data_as(_c_int32_p), i.size, p.size - 1,
+                                            ctypes.byref(out)))
+        else:
+            check(load().plk_expval_csr(self.h, _dp(d), i.ctypes.data_as(_c_int64_p),
+                                        p.ctypes.data_as(_c_int64_p), i.size, p.size - 1,
+                                        ctypes.byref(out)))
         return out.value
 
     def expval(self, ob: "Obs") -> float:
@@ -323,12 +345,17 @@ class Obs:
     @staticmethod
     def sparse(data, indices, indptr, wires: Sequence[int]) -> "Obs":
         d = _cplx_doubles(data)
-        i = np.ascontiguousarray(np.asarray(indices, dtype=np.int64).reshape(-1))
-        p = np.ascontiguousarray(np.asarray(indptr, dtype=np.int64).reshape(-1))
+        i, p, i32 = _csr_indices(indices, indptr)
         w = _sizes(wires)
         h = _vp()
-        check(load().plk_obs_sparse(_dp(d), i.ctypes.data_as(_c_int64_p), p.ctypes.data_as(_c_int64_p),
-                                    i.size, p.size - 1, _sp(w), w.size, ctypes.byref(h)))
+        if i32:
+            check(load().plk_obs_sparse_i32(_dp(d), i.ctypes.data_as(_c_int32_p),
+                                            p.ctypes.data_as(_c_int32_p), i.size, p.size - 1,
+                                            _sp(w), w.size, ctypes.byref(h)))
+        else:
+            check(load().plk_obs_sparse(_dp(d), i.ctypes.data_as(_c_int64_p),
+                                        p.ctypes.data_as(_c_int64_p), i.size, p.size - 1,
+                                        _sp(w), w.size, ctypes.byref(h)))
         return Obs(h)
 
     def wires(self) -> List[int]:

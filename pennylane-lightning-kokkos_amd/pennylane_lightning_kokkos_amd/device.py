@@ -188,7 +188,11 @@ class LightningKokkosAMD:
         if kind == "ham":
             return _capi.Obs.hamiltonian(desc[1], [self._obs(t) for t in desc[2]])
         if kind == "sparse":
-            return _capi.Obs.sparse(desc[1], desc[2], desc[3], self._map(desc[4]))
+            # Index width per precision as the reference serialises it
+            # (_serialize.py:136-154): int32 for complex64, int64 for complex128.
+            rtype = np.int32 if self.use_csingle else np.int64
+            return _capi.Obs.sparse(desc[1], np.asarray(desc[2]).astype(rtype),
+                                    np.asarray(desc[3]).astype(rtype), self._map(desc[4]))
         raise ValueError(f"unknown observable {kind}")
 
     def expval(self, observable) -> float:

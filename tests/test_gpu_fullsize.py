@@ -136,6 +136,13 @@ def test_c2_csr_20q(capi):
     e_pauli = s.expval(to_capi(capi, ham))
     assert abs(e_csr - e_ref) < 1e-10 * max(1.0, abs(e_ref))
     assert abs(e_pauli - e_ref) < 1e-10 * max(1.0, abs(e_ref))
+    # The complex64 serialisation: int32 indices, FP32 values on the device.
+    d32, i32, p32 = pauli_csr(n, ham, index_dtype=np.int32)
+    s64 = capi.State(n, np.complex64)
+    s64.upload(psi.astype(np.complex64))
+    e64 = s64.expval(capi.Obs.sparse(d32, i32, p32, list(range(n))))
+    assert abs(e64 - e_ref) < 1e-4 * max(1.0, abs(e_ref))
+    assert abs(s64.expval_csr(d32, i32, p32) - e_ref) < 1e-4 * max(1.0, abs(e_ref))
 
 
 def _layered_ops(n, layers, rng):
