@@ -168,6 +168,10 @@ struct PlanOp {
     CompiledGen G;
     int col = -1;
     double coef = 0.0;
+    int32_t src = -1;       // index of the op in its OpsData
+    int32_t param = -1;     // batch plans: global parameter index of a patched op
+    int16_t patch_gate = 0; // batch plans: BatchPatchGate of a patched op
+    bool adj = false;       // U was compiled with the inverse flag set
 };
 
 // Parity ops/generators on <= 2 wires become dense diagonals so they fuse.
@@ -764,9 +768,23 @@ SegMeta build_segment(const Segment &seg, int n, std::vector<unsigned char> &tab
     return meta;
 }
 
+// U payload layout a patched gate keeps for every angle (the structure of
+// the gate at a generic angle, which special angles only refine).
+int32_t patch_ukind(int16_t gate) {
+    switch (gate) {
+    case BPG_RX:
+        return U_DENSE;
+    case BPG_RY:
+        return U_REAL;
+    default:
+        return U_DIAG;
+    }
+}
+
 template <typename T>
 void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
-                std::vector<TileBundle> &btab, SegMeta &meta) {
+                std::vector<TileBundle> &btab, SegMeta &meta,
+                std::vector<BatchPatch> *patches = nullptr) {
     TileLaunch &TL = meta.TL;
     TL.n = n;
     TL.T = static_cast<int>(ps.tb.size());
@@ -809,11 +827,27 @@ void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
                 d.slot = slot++;
             }
             classify(d, w.size());
+            if (patches != nullptr && p.patch_gate != BPG_NONE) {
+                d.ukind = patch_ukind(p.patch_gate);
+            }
             group.push_back(d);
         }
         const size_t off = pack_ops(table, group.data(), group.size());
         if (off > static_cast<size_t>(INT32_MAX)) {
             raise(PLK_ERR_UNSUPPORTED, "operation table exceeds 2 GiB");
+        }
+        if (patches != nullptr) {
+            for (size_t o = 0; o < b.ops.size(); o++) {
+                const PlanOp &p = b.ops[o];
+                if (p.patch_gate == BPG_NONE) {
+                    continue;
+                }
+                OpHead h;
+                std::memcpy(&h, table.data() + off + o * sizeof(OpHead), sizeof(OpHead));
+                patches->push_back(BatchPatch{static_cast<int64_t>(off) + (h.offs & 0xffff),
+                                              p.param, p.patch_gate,
+                                              static_cast<int16_t>(p.adj ? 1 : 0)});
+            }
         }
         tb.first = static_cast<int32_t>(off);
         slot0 += b.nslots;
@@ -1536,6 +1570,8 @@ std::vector<PlanOp> reverse_ops(const OpsData &ops, const std::vector<size_t> &t
         PlanOp p;
         p.U = compile_op(ops.names[o], ops.wires[o], !ops.inverses[o], ops.params[o],
                          ops.matrices[o], n);
+        p.src = static_cast<int32_t>(o);
+        p.adj = !ops.inverses[o];
         if (!ops.params[o].empty()) {
             if (current_param_idx == *tp_it) {
                 p.trainable = true;
@@ -1726,8 +1762,36 @@ template <typename T> struct BatchPlan {
     std::vector<int32_t> slot_cols;
     std::vector<double> slot_coefs;
     std::vector<double> cost;
+    std::vector<BatchPatch> patches; // offsets into ops
     int64_t jac_len = 0;
+    int64_t nparams = 0; // parametric ops of the planned circuits
+
+    // Releases dst's memory (a merged part is no longer needed).
+    void swap_into(BatchPlan &dst) {
+        std::swap(*this, dst);
+    }
 };
+
+// Patched gate of an op: one-parameter rotations without an explicit matrix.
+int16_t patch_gate_of(const OpsData &ops, size_t i) {
+    if (ops.params[i].size() != 1 || !ops.matrices[i].empty()) {
+        return BPG_NONE;
+    }
+    const std::string &nm = ops.names[i];
+    if (nm == "RX") {
+        return BPG_RX;
+    }
+    if (nm == "RY") {
+        return BPG_RY;
+    }
+    if (nm == "RZ") {
+        return BPG_RZ;
+    }
+    if (nm == "PhaseShift") {
+        return BPG_PHASE;
+    }
+    return BPG_NONE;
+}
 
 template <typename T>
 void plan_batch_passes(BatchPlan<T> &bp, std::vector<Step> &&steps, const BatchCircuit &c,
@@ -1749,7 +1813,7 @@ void plan_batch_passes(BatchPlan<T> &bp, std::vector<Step> &&steps, const BatchC
         SegMeta meta;
         BatchPass P{};
         P.bundle_first = static_cast<int32_t>(bp.bundles.size());
-        build_pass<T>(st.pass, c.n, bp.ops, bp.bundles, meta);
+        build_pass<T>(st.pass, c.n, bp.ops, bp.bundles, meta, &bp.patches);
         P.T = meta.TL.T;
         P.bundle_count = meta.TL.nbundles;
         P.nslots = st.pass.nslots;
@@ -1814,6 +1878,33 @@ void plan_batch_circuit(BatchPlan<T> &bp, const BatchInput &in, int tile_bits, i
                          ops.matrices[i], n);
         fwd.push_back(std::move(p));
     }
+    // Global parameter index of every parametric op (circuit order), and the
+    // patch gate of the ones whose U payload is rewritten per call.
+    std::vector<int32_t> pidx(ops.names.size(), -1);
+    int64_t np = bp.nparams;
+    for (size_t i = 0; i < ops.names.size(); i++) {
+        if (!ops.params[i].empty()) {
+            pidx[i] = static_cast<int32_t>(np++);
+        }
+    }
+    if (np > INT32_MAX) {
+        raise(PLK_ERR_UNSUPPORTED, "batched parameters exceed 2^31");
+    }
+    auto mark = [&](PlanOp &p) {
+        if (p.src >= 0 && pidx[p.src] >= 0) {
+            p.param = pidx[p.src];
+            p.patch_gate = patch_gate_of(ops, static_cast<size_t>(p.src));
+        }
+    };
+    for (size_t i = 0; i < fwd.size(); i++) {
+        fwd[i].src = static_cast<int32_t>(i);
+        fwd[i].adj = ops.inverses[i];
+        mark(fwd[i]);
+    }
+    for (auto &p : rev) {
+        mark(p);
+    }
+    bp.nparams = np;
     plan_batch_passes(bp, plan_tiles(std::move(fwd), n, Tt, Lc, BATCH_PMAX), c, Tt, true,
                       c.fwd_first, c.fwd_count);
     plan_batch_passes(bp, plan_tiles(std::move(rev), n, Tt, Lc, BATCH_PMAX), c, Tt, false,
@@ -1866,27 +1957,159 @@ template <typename T> void merge_plan(BatchPlan<T> &dst, BatchPlan<T> &&src) {
     dst.slot_cols.insert(dst.slot_cols.end(), src.slot_cols.begin(), src.slot_cols.end());
     dst.slot_coefs.insert(dst.slot_coefs.end(), src.slot_coefs.begin(), src.slot_coefs.end());
     dst.cost.insert(dst.cost.end(), src.cost.begin(), src.cost.end());
+    for (auto p : src.patches) {
+        p.off += static_cast<int64_t>(ops_off);
+        p.param += static_cast<int32_t>(dst.nparams);
+        dst.patches.push_back(p);
+    }
     dst.jac_len += src.jac_len;
+    dst.nparams += src.nparams;
 }
 
-template <typename T>
-void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double *expvals,
-                       double *jac) {
-    const int dtype = sizeof(T) == 4 ? PLK_C64 : PLK_C128;
-    const int tile_bits = batch_tile_bits(dtype);
-    if (in.empty()) {
+// Runs fn(t, lo, hi) for nthreads contiguous ranges of [0, count) on host
+// threads (inline when there is one).
+template <typename F> void parallel_ranges(size_t count, size_t nthreads, F fn) {
+    nthreads = std::max<size_t>(1, std::min(nthreads, count));
+    if (nthreads == 1) {
+        fn(size_t{0}, size_t{0}, count);
         return;
     }
-    // Circuits are planned independently on host threads, in contiguous
-    // chunks, and the chunk plans are concatenated in circuit order.
+    std::vector<std::thread> pool;
+    for (size_t t = 0; t < nthreads; t++) {
+        pool.emplace_back(fn, t, count * t / nthreads, count * (t + 1) / nthreads);
+    }
+    for (auto &th : pool) {
+        th.join();
+    }
+}
+
+size_t plan_threads(size_t count) {
     const size_t hw = std::max(1u, std::thread::hardware_concurrency());
-    const size_t nthreads = std::min<size_t>(
+    return std::min<size_t>(
         {static_cast<size_t>(std::max(1, env_int("PLK_PLAN_THREADS", static_cast<int>(hw)))),
-         (in.size() + 31) / 32, size_t{64}});
+         (count + 31) / 32, size_t{64}});
+}
+
+// 64-bit structure hash (multiply-xorshift over 64-bit words).
+struct Hasher {
+    uint64_t h = 0x243F6A8885A308D3ull;
+    void add(uint64_t v) {
+        h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h *= 0xFF51AFD7ED558CCDull;
+        h ^= h >> 32;
+    }
+    void add(double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, sizeof b);
+        add(b);
+    }
+    void add(const std::string &str) {
+        add(static_cast<uint64_t>(str.size()));
+        add(static_cast<uint64_t>(std::hash<std::string>{}(str)));
+    }
+};
+
+void hash_obs(Hasher &H, const ObsNode &o) {
+    H.add(static_cast<uint64_t>(o.kind));
+    H.add(o.name);
+    H.add(static_cast<uint64_t>(o.wires.size()));
+    for (auto w : o.wires) {
+        H.add(static_cast<uint64_t>(w));
+    }
+    H.add(static_cast<uint64_t>(o.coeffs.size()));
+    for (double c : o.coeffs) {
+        H.add(c);
+    }
+    H.add(static_cast<uint64_t>(o.matrix.size()));
+    for (const cd &m : o.matrix) {
+        H.add(m.real());
+        H.add(m.imag());
+    }
+    H.add(static_cast<uint64_t>(o.terms.size()));
+    for (const auto &t : o.terms) {
+        hash_obs(H, *t);
+    }
+}
+
+// Everything a batch plan depends on except the angles of patched gates:
+// qubits, op names / wires / inverse flags, the values of every other
+// parameter and matrix, the trainable set and the observable.
+uint64_t circuit_key(const BatchInput &in) {
+    Hasher H;
+    const OpsData &ops = *in.ops;
+    H.add(static_cast<uint64_t>(in.n));
+    H.add(static_cast<uint64_t>(ops.names.size()));
+    for (size_t i = 0; i < ops.names.size(); i++) {
+        H.add(ops.names[i]);
+        H.add(static_cast<uint64_t>(ops.wires[i].size() | (ops.inverses[i] ? size_t{1} << 40 : 0)));
+        for (auto w : ops.wires[i]) {
+            H.add(static_cast<uint64_t>(w));
+        }
+        H.add(static_cast<uint64_t>(ops.params[i].size()));
+        if (patch_gate_of(ops, i) == BPG_NONE) {
+            for (double v : ops.params[i]) {
+                H.add(v);
+            }
+        }
+        H.add(static_cast<uint64_t>(ops.matrices[i].size()));
+        for (const cd &m : ops.matrices[i]) {
+            H.add(m.real());
+            H.add(m.imag());
+        }
+    }
+    H.add(static_cast<uint64_t>(in.tp.size()));
+    for (auto t : in.tp) {
+        H.add(static_cast<uint64_t>(t));
+    }
+    hash_obs(H, *in.obs);
+    return H.h;
+}
+
+// The last batch plan of one precision, resident on its device: tables,
+// patch list, parameter slots, scratch and result buffers in one allocation.
+// A call whose circuits have the same keys reuses it and uploads only the
+// parameters; any other call replaces it.
+template <typename T> struct BatchCache {
+    bool valid = false;
+    int device = -1;
+    int tile_bits = 0;
+    std::vector<uint64_t> keys;
+    std::vector<int64_t> param_base; // first global parameter of each circuit
+    int64_t nparams = 0;
+    int64_t npatches = 0;
+    void *dbuf = nullptr;
+    void *ops = nullptr;
+    const BatchPatch *patches = nullptr;
+    double *params = nullptr;
+    BatchArgs A{};
+
+    void release() {
+        valid = false;
+        if (dbuf != nullptr) {
+            PLK_HIP(hipStreamSynchronize(device_stream()));
+            PLK_HIP(hipFree(dbuf));
+            dbuf = nullptr;
+        }
+    }
+    hipStream_t device_stream() const { return ::plk::device(device).stream; }
+};
+
+template <typename T> BatchCache<T> &batch_cache() {
+    static BatchCache<T> c; // released with the process
+    return c;
+}
+
+// Plans every circuit and makes the result resident in C.
+// Every validation error is raised while planning, before any device call.
+template <typename T>
+void build_batch_cache(BatchCache<T> &C, int device_id, const std::vector<BatchInput> &in,
+                       int tile_bits) {
+    const int dtype = sizeof(T) == 4 ? PLK_C64 : PLK_C128;
+    const size_t nthreads = plan_threads(in.size());
+    std::unique_ptr<HostScope> phase = std::make_unique<HostScope>("host_batch_plan");
     std::vector<BatchPlan<T>> parts(nthreads);
     std::vector<std::pair<size_t, Error>> failures(nthreads, {SIZE_MAX, Error{PLK_OK, ""}});
-    auto work = [&](size_t t) {
-        const size_t lo = in.size() * t / nthreads, hi = in.size() * (t + 1) / nthreads;
+    parallel_ranges(in.size(), nthreads, [&](size_t t, size_t lo, size_t hi) {
         for (size_t i = lo; i < hi; i++) {
             try {
                 plan_batch_circuit<T>(parts[t], in[i], tile_bits, low_bits(dtype));
@@ -1898,28 +2121,41 @@ void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double 
                 return;
             }
         }
-    };
-    if (nthreads == 1) {
-        work(0);
-    } else {
-        std::vector<std::thread> pool;
-        for (size_t t = 0; t < nthreads; t++) {
-            pool.emplace_back(work, t);
-        }
-        for (auto &th : pool) {
-            th.join();
-        }
-    }
+    });
     for (const auto &f : failures) {
         if (f.first != SIZE_MAX) {
             raise(f.second.code, "circuit " + std::to_string(f.first) + ": " + f.second.msg);
         }
     }
+    phase = std::make_unique<HostScope>("host_batch_merge");
     BatchPlan<T> bp = std::move(parts[0]);
-    for (size_t t = 1; t < nthreads; t++) {
-        merge_plan(bp, std::move(parts[t]));
+    {
+        // One allocation per table instead of a regrowth per part.
+        size_t nc = 0, np = 0, nb = 0, no = 0, nt = 0, ns = 0, npt = 0;
+        for (const auto &pt : parts) {
+            nc += pt.circuits.size();
+            np += pt.passes.size();
+            nb += pt.bundles.size();
+            no += pt.ops.size() + 16;
+            nt += pt.terms.size();
+            ns += pt.slot_cols.size();
+            npt += pt.patches.size();
+        }
+        bp.circuits.reserve(nc);
+        bp.cost.reserve(nc);
+        bp.passes.reserve(np);
+        bp.bundles.reserve(nb);
+        bp.ops.reserve(no);
+        bp.terms.reserve(nt);
+        bp.slot_cols.reserve(ns);
+        bp.slot_coefs.reserve(ns);
+        bp.patches.reserve(npt);
     }
-    set_device(device_id);
+    for (size_t t = 1; t < parts.size(); t++) {
+        merge_plan(bp, std::move(parts[t]));
+        BatchPlan<T>().swap_into(parts[t]);
+    }
+    phase = std::make_unique<HostScope>("host_batch_pack");
     Device &dev = device(device_id);
 
     // Longest-processing-time-first packing of circuits onto workgroups.
@@ -1972,17 +2208,25 @@ void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double 
     const size_t o_terms = blob.add(bp.terms);
     const size_t o_cols = blob.add(bp.slot_cols);
     const size_t o_coefs = blob.add(bp.slot_coefs);
-    const size_t tables = (blob.bytes.size() + 255) & ~size_t{255};
+    const size_t o_patch = blob.add(bp.patches);
+    auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
     const size_t amp = 2 * sizeof(T);
-    const size_t o_scratch = tables;
-    const size_t o_exp = o_scratch + ((static_cast<size_t>(scratch_amps) * amp + 255) & ~size_t{255});
-    const size_t o_jac = o_exp + ((in.size() * sizeof(double) + 255) & ~size_t{255});
+    const size_t o_params = up(blob.bytes.size());
+    const size_t o_scratch = o_params + up(static_cast<size_t>(bp.nparams) * sizeof(double));
+    const size_t o_exp = o_scratch + up(static_cast<size_t>(scratch_amps) * amp);
+    const size_t o_jac = o_exp + up(in.size() * sizeof(double));
     const size_t total = o_jac + static_cast<size_t>(bp.jac_len) * sizeof(double) + 256;
-    unsigned char *w = static_cast<unsigned char *>(ensure_work(dev, total));
-    PLK_HIP(hipMemcpyAsync(w, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice,
-                           dev.stream));
 
-    BatchArgs A{};
+    phase = std::make_unique<HostScope>("host_batch_upload");
+    C.release();
+    C.device = dev.id;
+    C.dbuf = dev_malloc(dev.id, total, "batch plan");
+    auto *w = static_cast<unsigned char *>(C.dbuf);
+    PLK_HIP(hipMemcpy(w, blob.bytes.data(), blob.bytes.size(), hipMemcpyHostToDevice));
+    record_kernel("host_batch_table_bytes", 0.f, static_cast<double>(blob.bytes.size()));
+
+    BatchArgs &A = C.A;
+    A = BatchArgs{};
     A.circuits = reinterpret_cast<const BatchCircuit *>(w + o_circ);
     A.order = reinterpret_cast<const int32_t *>(w + o_order);
     A.block_first = reinterpret_cast<const int32_t *>(w + o_bfirst);
@@ -1999,13 +2243,74 @@ void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double 
     A.jac_len = bp.jac_len;
     A.tile_bits = tile_bits;
     A.grid = grid;
-    launch_batch(dev, dtype, A);
-    PLK_HIP(hipMemcpyAsync(expvals, A.expvals, in.size() * sizeof(double), hipMemcpyDeviceToHost,
-                           dev.stream));
-    if (bp.jac_len > 0) {
-        PLK_HIP(hipMemcpyAsync(jac, A.jac, static_cast<size_t>(bp.jac_len) * sizeof(double),
+    C.ops = w + o_ops;
+    C.patches = reinterpret_cast<const BatchPatch *>(w + o_patch);
+    C.npatches = static_cast<int64_t>(bp.patches.size());
+    C.params = reinterpret_cast<double *>(w + o_params);
+    C.nparams = bp.nparams;
+    C.tile_bits = tile_bits;
+    C.param_base.assign(in.size(), 0);
+    int64_t base = 0;
+    for (size_t i = 0; i < in.size(); i++) {
+        C.param_base[i] = base;
+        base += static_cast<int64_t>(in[i].ops->num_par_ops());
+    }
+    C.valid = true;
+}
+
+template <typename T>
+void adjoint_batched_t(int device_id, const std::vector<BatchInput> &in, double *expvals,
+                       double *jac) {
+    const int dtype = sizeof(T) == 4 ? PLK_C64 : PLK_C128;
+    const int tile_bits = batch_tile_bits(dtype);
+    if (in.empty()) {
+        return;
+    }
+    const size_t nthreads = plan_threads(in.size());
+    std::unique_ptr<HostScope> phase = std::make_unique<HostScope>("host_batch_key");
+    std::vector<uint64_t> keys(in.size());
+    parallel_ranges(in.size(), nthreads, [&](size_t, size_t lo, size_t hi) {
+        for (size_t i = lo; i < hi; i++) {
+            keys[i] = circuit_key(in[i]);
+        }
+    });
+    phase.reset();
+    BatchCache<T> &C = batch_cache<T>();
+    if (!(C.valid && C.device == device_id && C.tile_bits == tile_bits && C.keys == keys)) {
+        C.valid = false;
+        build_batch_cache<T>(C, device_id, in, tile_bits);
+        C.keys = std::move(keys);
+    }
+    Device &dev = device(device_id);
+
+    // Angles of this call, in the plan's parameter order, then the patch
+    // kernel rewrites every patched U payload on the device.
+    phase = std::make_unique<HostScope>("host_batch_params");
+    std::vector<double> params(static_cast<size_t>(C.nparams));
+    parallel_ranges(in.size(), nthreads, [&](size_t, size_t lo, size_t hi) {
+        for (size_t c = lo; c < hi; c++) {
+            const OpsData &ops = *in[c].ops;
+            int64_t k = C.param_base[c];
+            for (size_t i = 0; i < ops.params.size(); i++) {
+                if (!ops.params[i].empty()) {
+                    params[static_cast<size_t>(k++)] = ops.params[i][0];
+                }
+            }
+        }
+    });
+    PLK_HIP(hipMemcpyAsync(C.params, params.data(), params.size() * sizeof(double),
+                           hipMemcpyHostToDevice, dev.stream));
+    phase = std::make_unique<HostScope>("host_batch_launch");
+    launch_batch_patch(dev, dtype, C.ops, C.patches, C.npatches, C.params);
+    launch_batch(dev, dtype, C.A);
+    phase = std::make_unique<HostScope>("host_batch_results");
+    PLK_HIP(hipMemcpyAsync(expvals, C.A.expvals, in.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, dev.stream));
+    if (C.A.jac_len > 0) {
+        PLK_HIP(hipMemcpyAsync(jac, C.A.jac, static_cast<size_t>(C.A.jac_len) * sizeof(double),
                                hipMemcpyDeviceToHost, dev.stream));
     }
+    // Also keeps `params` alive until its upload has been read.
     PLK_HIP(hipStreamSynchronize(dev.stream));
 }
 
@@ -2563,33 +2868,37 @@ int plk_adjoint_batched(int dtype, int device_id, size_t num_circuits, const int
         if (trainable && !num_trainable) {
             raise(PLK_ERR_INVALID, "trainable lists need their lengths");
         }
+        HostScope whole("host_batch_call");
         std::vector<BatchInput> in(num_circuits);
+        std::unique_ptr<HostScope> inputs = std::make_unique<HostScope>("host_batch_inputs");
         for (size_t i = 0; i < num_circuits; i++) {
-            const std::string who = "circuit " + std::to_string(i) + ": ";
+            auto who = [i]() { return "circuit " + std::to_string(i) + ": "; };
             if (num_qubits[i] < 1 || num_qubits[i] > BATCH_NMAX) {
-                raise(PLK_ERR_UNSUPPORTED, who + "batched circuits have 1.." +
+                raise(PLK_ERR_UNSUPPORTED, who() + "batched circuits have 1.." +
                                                std::to_string(BATCH_NMAX) + " qubits");
             }
             if (!ops[i]) {
-                raise(PLK_ERR_INVALID, who + "null ops handle");
+                raise(PLK_ERR_INVALID, who() + "null ops handle");
             }
             in[i].n = num_qubits[i];
             in[i].ops = &ops[i]->data;
             in[i].obs = &O(obs[i]);
             if (trainable) {
                 if (num_trainable[i] > 0 && !trainable[i]) {
-                    raise(PLK_ERR_INVALID, who + "null trainable list");
+                    raise(PLK_ERR_INVALID, who() + "null trainable list");
                 }
                 in[i].tp.assign(trainable[i], trainable[i] + num_trainable[i]);
             } else {
-                for (size_t k = 0; k < ops[i]->data.num_par_ops(); k++) {
-                    in[i].tp.push_back(k);
+                in[i].tp.resize(ops[i]->data.num_par_ops());
+                for (size_t k = 0; k < in[i].tp.size(); k++) {
+                    in[i].tp[k] = k;
                 }
             }
             if (in[i].tp.empty()) {
-                raise(PLK_ERR_INVALID, who + "No trainable parameters provided.");
+                raise(PLK_ERR_INVALID, who() + "No trainable parameters provided.");
             }
         }
+        inputs.reset();
         if (num_circuits > 0 && !jac) {
             raise(PLK_ERR_INVALID, "null jac buffer");
         }

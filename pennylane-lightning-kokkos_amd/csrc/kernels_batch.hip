@@ -331,6 +331,74 @@ int batch_resident_blocks(int dtype, int tile_bits) {
     return std::max(1, std::min(nb, 8));
 }
 
+namespace {
+
+// One thread per patch: t = +-params[param], c = cos(t/2), s = sin(t/2).
+template <typename T>
+__global__ void __launch_bounds__(256)
+    batch_patch_kernel(unsigned char *__restrict__ ops, const BatchPatch *__restrict__ patches,
+                       int64_t count, const double *__restrict__ params) {
+    const int64_t i = int64_t{blockIdx.x} * blockDim.x + threadIdx.x;
+    if (i >= count) {
+        return;
+    }
+    const BatchPatch p = patches[i];
+    const double t = p.adj ? -params[p.param] : params[p.param];
+    T *u = reinterpret_cast<T *>(ops + p.off);
+    double s, c;
+    if (p.gate == BPG_PHASE) {
+        sincos(t, &s, &c);
+        u[0] = T(1);
+        u[1] = T(0);
+        u[2] = static_cast<T>(c);
+        u[3] = static_cast<T>(s);
+        return;
+    }
+    sincos(0.5 * t, &s, &c);
+    if (p.gate == BPG_RX) {
+        // [[c, -i s], [-i s, c]], dense complex row-major.
+        u[0] = static_cast<T>(c);
+        u[1] = T(0);
+        u[2] = T(0);
+        u[3] = static_cast<T>(-s);
+        u[4] = T(0);
+        u[5] = static_cast<T>(-s);
+        u[6] = static_cast<T>(c);
+        u[7] = T(0);
+    } else if (p.gate == BPG_RY) {
+        // [[c, -s], [s, c]], real row-major.
+        u[0] = static_cast<T>(c);
+        u[1] = static_cast<T>(-s);
+        u[2] = static_cast<T>(s);
+        u[3] = static_cast<T>(c);
+    } else if (p.gate == BPG_RZ) {
+        // diag(e^{-i t/2}, e^{i t/2}), complex diagonal.
+        u[0] = static_cast<T>(c);
+        u[1] = static_cast<T>(-s);
+        u[2] = static_cast<T>(c);
+        u[3] = static_cast<T>(s);
+    }
+}
+
+} // namespace
+
+void launch_batch_patch(Device &dev, int dtype, void *ops, const BatchPatch *patches,
+                        int64_t count, const double *params) {
+    if (count <= 0) {
+        return;
+    }
+    const unsigned grid = static_cast<unsigned>((count + 255) / 256);
+    ProfScope prof("batch_patch", 0.0, dev.stream);
+    if (dtype == PLK_C64) {
+        hipLaunchKernelGGL(batch_patch_kernel<float>, dim3(grid), dim3(256), 0, dev.stream,
+                           static_cast<unsigned char *>(ops), patches, count, params);
+    } else {
+        hipLaunchKernelGGL(batch_patch_kernel<double>, dim3(grid), dim3(256), 0, dev.stream,
+                           static_cast<unsigned char *>(ops), patches, count, params);
+    }
+    PLK_HIP(hipGetLastError());
+}
+
 void launch_batch(Device &dev, int dtype, const BatchArgs &A) {
     if (dtype == PLK_C64) {
         launch_t<float>(dev, A);

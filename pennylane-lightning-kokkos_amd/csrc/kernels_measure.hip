@@ -199,10 +199,14 @@ __global__ void __launch_bounds__(kBlock)
 // CSR kernels: one wave works on R = 64 / G consecutive rows at a time, G
 // lanes per row striding over the row's entries, so a wave's loads of the
 // values and indices are contiguous runs (streamed with non-temporal loads:
-// they are read once), while the gathers psi[col] stay cacheable. Rows are
-// dealt to waves round-robin. Every row range and column index is checked:
-// a malformed matrix is skipped and counted in the block's flag slot
-// (partials[grid + block]) instead of being read out of bounds.
+// they are read once), while the gathers psi[col] stay cacheable. Each block
+// owns one contiguous range of `chunk` rows and its four waves take
+// consecutive row groups of it: rows that gather from the same cache lines
+// (a Pauli-sum matrix has col = row ^ x, so 8 consecutive complex128 rows
+// share every line) run on one CU at about the same time instead of being
+// spread over the XCDs' separate L2s. Every row range and column index is
+// checked: a malformed matrix is skipped and counted in the block's flag slot
+// instead of being read out of bounds.
 template <typename V> struct Vec2;
 template <> struct Vec2<float> {
     typedef float type __attribute__((ext_vector_type(2)));
@@ -226,14 +230,15 @@ __global__ void __launch_bounds__(kBlock)
     csr_expval_kernel(const Cx<T> *__restrict__ psi, uint64_t len,
                       const Cx<V> *__restrict__ data, const I *__restrict__ indices,
                       const I *__restrict__ indptr, uint64_t rows, uint64_t nnz,
-                      double *__restrict__ partials) {
+                      uint64_t chunk, double *__restrict__ partials) {
     constexpr int R = 64 / G;
+    constexpr int W = kBlock / 64;
     const int lane = threadIdx.x & 63;
     const int sub = lane & (G - 1);
-    const uint64_t wave = uint64_t{blockIdx.x} * (kBlock / 64) + (threadIdx.x >> 6);
-    const uint64_t nwaves = uint64_t{gridDim.x} * (kBlock / 64);
+    const uint64_t first = uint64_t{blockIdx.x} * chunk;
+    const uint64_t last = first + chunk < rows ? first + chunk : rows;
     double acc[2] = {0.0, 0.0};
-    for (uint64_t r = wave * R + lane / G; r < rows; r += nwaves * R) {
+    for (uint64_t r = first + (threadIdx.x >> 6) * R + lane / G; r < last; r += W * R) {
         const int64_t start = indptr[r], end = indptr[r + 1];
         if (start < 0 || end < start || static_cast<uint64_t>(end) > nnz) {
             acc[1] = 1.0;
@@ -268,18 +273,19 @@ __global__ void __launch_bounds__(kBlock)
     csr_spmv_kernel(Cx<T> *__restrict__ out, const Cx<T> *__restrict__ psi, uint64_t len,
                     const Cx<V> *__restrict__ data, const I *__restrict__ indices,
                     const I *__restrict__ indptr, uint64_t rows, uint64_t nnz,
-                    double *__restrict__ partials) {
+                    uint64_t chunk, double *__restrict__ partials) {
     constexpr int R = 64 / G;
+    constexpr int W = kBlock / 64;
     const int lane = threadIdx.x & 63;
     const int sub = lane & (G - 1);
-    const uint64_t wave = uint64_t{blockIdx.x} * (kBlock / 64) + (threadIdx.x >> 6);
-    const uint64_t nwaves = uint64_t{gridDim.x} * (kBlock / 64);
+    const uint64_t first = uint64_t{blockIdx.x} * chunk;
+    const uint64_t last = first + chunk < rows ? first + chunk : rows;
     double bad[1] = {0.0};
     // Wave-uniform loop: every lane takes part in the row-sum shuffles.
-    for (uint64_t r0 = wave * R; r0 < rows; r0 += nwaves * R) {
+    for (uint64_t r0 = first + (threadIdx.x >> 6) * R; r0 < last; r0 += W * R) {
         const uint64_t r = r0 + lane / G;
         double sr = 0.0, si = 0.0;
-        if (r < rows) {
+        if (r < last) {
             const int64_t start = indptr[r], end = indptr[r + 1];
             if (start < 0 || end < start || static_cast<uint64_t>(end) > nnz) {
                 bad[0] = 1.0;
@@ -303,7 +309,7 @@ __global__ void __launch_bounds__(kBlock)
             sr += __shfl_xor(sr, off, 64);
             si += __shfl_xor(si, off, 64);
         }
-        if (sub == 0 && r < rows) {
+        if (sub == 0 && r < last) {
             st(out, r, Cx<T>{static_cast<T>(sr), static_cast<T>(si)});
         }
     }
@@ -602,11 +608,15 @@ int csr_lanes(size_t nnz, size_t rows) {
     return avg <= 6.0 ? 4 : (avg <= 24.0 ? 16 : 64);
 }
 
-unsigned csr_grid(const Device &dev, size_t rows, int G) {
-    const uint64_t rows_per_block = static_cast<uint64_t>(64 / G) * (kBlock / 64);
-    const uint64_t want = (rows + rows_per_block - 1) / rows_per_block;
+// Grid and rows per block: contiguous ranges, a multiple of the rows one
+// block iteration covers, at most 8 blocks per CU.
+void csr_grid(const Device &dev, size_t rows, int G, unsigned &grid, uint64_t &chunk) {
+    const uint64_t step = static_cast<uint64_t>(64 / G) * (kBlock / 64);
+    const uint64_t groups = (rows + step - 1) / step;
     const uint64_t cap = static_cast<uint64_t>(dev.num_cus) * 8;
-    return static_cast<unsigned>(std::max<uint64_t>(1, std::min(want, cap)));
+    const uint64_t blocks = std::max<uint64_t>(1, std::min(groups, cap));
+    chunk = std::max<uint64_t>(1, (groups + blocks - 1) / blocks) * step;
+    grid = static_cast<unsigned>(std::max<uint64_t>(1, (rows + chunk - 1) / chunk));
 }
 
 // Raw device views of one CSR launch.
@@ -618,7 +628,7 @@ struct CsrArgs {
 
 template <typename T, typename V, typename I>
 void csr_launch_t(Device &dev, const CsrArgs &m, const State &in, Cx<T> *out, unsigned grid,
-                  int G, double *partials) {
+                  uint64_t chunk, int G, double *partials) {
     const auto *psi = static_cast<const Cx<T> *>(in.d);
     const auto *d = static_cast<const Cx<V> *>(m.data);
     const auto *ix = static_cast<const I *>(m.indices);
@@ -627,10 +637,10 @@ void csr_launch_t(Device &dev, const CsrArgs &m, const State &in, Cx<T> *out, un
 #define PLK_CSR_LAUNCH(GG)                                                                        \
     if (out == nullptr) {                                                                         \
         hipLaunchKernelGGL((csr_expval_kernel<T, V, I, GG>), dim3(grid), dim3(kBlock), 0,       \
-                           dev.stream, psi, len, d, ix, ip, rows, nnz, partials);                 \
+                           dev.stream, psi, len, d, ix, ip, rows, nnz, chunk, partials);          \
     } else {                                                                                      \
         hipLaunchKernelGGL((csr_spmv_kernel<T, V, I, GG>), dim3(grid), dim3(kBlock), 0,         \
-                           dev.stream, out, psi, len, d, ix, ip, rows, nnz, partials);            \
+                           dev.stream, out, psi, len, d, ix, ip, rows, nnz, chunk, partials);     \
     }
     if (G == 4) {
         PLK_CSR_LAUNCH(4)
@@ -648,7 +658,9 @@ void csr_launch_t(Device &dev, const CsrArgs &m, const State &in, Cx<T> *out, un
 std::pair<double, double> csr_run(Device &dev, const CsrArgs &m, const State &in,
                                   const State *out) {
     const int G = csr_lanes(m.nnz, m.rows);
-    const unsigned grid = csr_grid(dev, m.rows, G);
+    unsigned grid = 1;
+    uint64_t chunk = 1;
+    csr_grid(dev, m.rows, G, grid, chunk);
     ensure_red(dev, static_cast<size_t>(grid) * 2, 2);
     {
         const double vb = m.vdouble ? 16.0 : 8.0;
@@ -661,14 +673,14 @@ std::pair<double, double> csr_run(Device &dev, const CsrArgs &m, const State &in
             auto *o = out ? static_cast<Cx<float> *>(out->d) : nullptr;
             if (m.vdouble) {
                 if (m.idx32) {
-                    csr_launch_t<float, double, int32_t>(dev, m, in, o, grid, G, dev.red);
+                    csr_launch_t<float, double, int32_t>(dev, m, in, o, grid, chunk, G, dev.red);
                 } else {
-                    csr_launch_t<float, double, int64_t>(dev, m, in, o, grid, G, dev.red);
+                    csr_launch_t<float, double, int64_t>(dev, m, in, o, grid, chunk, G, dev.red);
                 }
             } else if (m.idx32) {
-                csr_launch_t<float, float, int32_t>(dev, m, in, o, grid, G, dev.red);
+                csr_launch_t<float, float, int32_t>(dev, m, in, o, grid, chunk, G, dev.red);
             } else {
-                csr_launch_t<float, float, int64_t>(dev, m, in, o, grid, G, dev.red);
+                csr_launch_t<float, float, int64_t>(dev, m, in, o, grid, chunk, G, dev.red);
             }
         } else {
             if (!m.vdouble) {
@@ -676,9 +688,9 @@ std::pair<double, double> csr_run(Device &dev, const CsrArgs &m, const State &in
             }
             auto *o = out ? static_cast<Cx<double> *>(out->d) : nullptr;
             if (m.idx32) {
-                csr_launch_t<double, double, int32_t>(dev, m, in, o, grid, G, dev.red);
+                csr_launch_t<double, double, int32_t>(dev, m, in, o, grid, chunk, G, dev.red);
             } else {
-                csr_launch_t<double, double, int64_t>(dev, m, in, o, grid, G, dev.red);
+                csr_launch_t<double, double, int64_t>(dev, m, in, o, grid, chunk, G, dev.red);
             }
         }
     }

@@ -85,6 +85,16 @@ struct ProfScope {
     ~ProfScope();
 };
 
+// Host wall time of a scope, recorded as one launch of `family` (no bytes)
+// while profiling is on: host planning, packing and copies next to the
+// kernels they feed.
+struct HostScope {
+    const char *family;
+    double t0 = -1.0;
+    explicit HostScope(const char *f);
+    ~HostScope();
+};
+
 // ---------------------------------------------------------------------------
 // State vector.
 // ---------------------------------------------------------------------------
@@ -354,6 +364,22 @@ struct BatchArgs {
     int32_t tile_bits;
     int32_t grid;
 };
+
+// Parameter patch of a resident batch plan: the U payload of one packed op at
+// byte `off` of the op table is rewritten from params[param] (negated when
+// adj) by the gate's formula, in the payload layout its ukind implies
+// (RX dense, RY real, RZ and PhaseShift diagonal). Circuits whose structure
+// repeats between calls (a training loop) reuse the plan and upload only
+// their parameters.
+enum BatchPatchGate : int32_t { BPG_NONE = 0, BPG_RX = 1, BPG_RY = 2, BPG_RZ = 3, BPG_PHASE = 4 };
+struct BatchPatch {
+    int64_t off;
+    int32_t param;
+    int16_t gate; // BatchPatchGate
+    int16_t adj;
+};
+void launch_batch_patch(Device &dev, int dtype, void *ops, const BatchPatch *patches,
+                        int64_t count, const double *params);
 
 size_t batch_lds_bytes(int dtype, int tile_bits);
 // Workgroups of the batch kernel resident per CU at this tile width.

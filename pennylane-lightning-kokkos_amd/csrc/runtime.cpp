@@ -6,6 +6,8 @@
 #include <unordered_map>
 #include <vector>
 
+#include <chrono>
+
 #include "plk_internal.hpp"
 
 namespace plk {
@@ -160,6 +162,26 @@ ProfScope::~ProfScope() {
     }
     std::lock_guard<std::mutex> lock(g_prof_mutex);
     g_pending.push_back({family, bytes, e0, e1});
+}
+
+namespace {
+double host_ms() {
+    return std::chrono::duration<double, std::milli>(
+               std::chrono::steady_clock::now().time_since_epoch())
+        .count();
+}
+} // namespace
+
+HostScope::HostScope(const char *f) : family(f) {
+    if (g_prof_on) {
+        t0 = host_ms();
+    }
+}
+
+HostScope::~HostScope() {
+    if (t0 >= 0.0) {
+        record_kernel(family, static_cast<float>(host_ms() - t0), 0.0);
+    }
 }
 
 void record_kernel(const char *family, float ms, double bytes) {

@@ -394,6 +394,7 @@ class Ops:
                                     _dp(mflat), _sp(moff), ctypes.byref(h)))
         self.h = h
         self.names = list(names)
+        self._num_par = int(np.count_nonzero(np.diff(poff)))
 
     def __del__(self):
         try:
@@ -403,7 +404,11 @@ class Ops:
         except Exception:  # pragma: no cover
             pass
 
-    def num_par_ops(self) -> int:
+    def num_par_ops(self, query: bool = False) -> int:
+        """getNumParOps (ops with a non-empty parameter list); counted at
+        creation unless query asks the library."""
+        if not query:
+            return self._num_par
         c = ctypes.c_size_t()
         check(load().plk_ops_num_par_ops(self.h, ctypes.byref(c)))
         return c.value
@@ -447,10 +452,8 @@ def adjoint_batched(num_qubits: Sequence[int], ops: Sequence[Ops], observables: 
     jac = np.zeros(max(1, total), dtype=np.float64)
     check(load().plk_adjoint_batched(code, int(device_id), nc, nq, oarr, barr, tarr,
                                      None if ntr is None else _sp(ntr), _dp(expvals), _dp(jac)))
-    rows, off = [], 0
-    for L in lens:
-        rows.append(jac[off:off + L].copy())
-        off += L
+    offs = np.concatenate(([0], np.cumsum(lens, dtype=np.int64))).tolist()
+    rows = [jac[a:b] for a, b in zip(offs[:-1], offs[1:])]
     return expvals[:nc], rows
 
 

@@ -147,3 +147,45 @@ def test_batched_large_streamed_circuits(capi, dtype):
     circuits, tps = _random_batch(rng, 4, 17, 20, nops=50)
     small, stps = _random_batch(rng, 6, 4, 9, nops=20)
     _check(capi, circuits + small, tps + stps, dtype)
+
+
+def _reparam(circuits, rng, special=None):
+    """The same circuits with fresh angles (or `special` for every angle)."""
+    out = []
+    for n, ops, ob in circuits:
+        params = [[float(special if special is not None else rng.uniform(0, 2 * np.pi))] * len(p)
+                  if p else [] for p in ops[1]]
+        out.append((n, (ops[0], params, ops[2], ops[3], ops[4]), ob))
+    return out
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_batched_plan_reuse_new_angles(capi, dtype):
+    """Repeated structure reuses the resident plan and only re-uploads the
+    angles: each call must match the oracle for its own angles, including a
+    first call at angle 0 (where RY / RX are diagonal) followed by generic
+    angles, and a change of every other parameter or observable (replanned)."""
+    rng = np.random.default_rng(7300)
+    circuits, tps = _random_batch(rng, 24, 2, 9)
+    _check(capi, _reparam(circuits, rng, special=0.0), tps, dtype)
+    for _ in range(2):
+        _check(capi, _reparam(circuits, rng), tps, dtype)
+    # Observable coefficients changed: a different plan.
+    changed = [(n, ops, ("ham", [0.5 * (k + 1) for k in range(n)], [("named", "PauliZ", [w]) for w in range(n)]))
+               for n, ops, _ in circuits]
+    _check(capi, changed, tps, dtype)
+    _check(capi, _reparam(changed, rng), tps, dtype)
+    # Trainable set changed.
+    tps2 = [tp[:1] for tp in tps]
+    _check(capi, _reparam(changed, rng), tps2, dtype)
+
+
+def test_batched_plan_reuse_c4_sample(capi):
+    """The C4 workload's structure (RY / RZ layers, CNOT chain) with two
+    angle sets in a row, complex64 and complex128 interleaved."""
+    rng = np.random.default_rng(7400)
+    circuits = [(n, ops, ob) for n, ops, ob in c4_workload(count=24, seed=99)]
+    for _ in range(2):
+        fresh = _reparam(circuits, rng)
+        _check(capi, fresh, None, np.complex64)
+        _check(capi, fresh, None, np.complex128)
