@@ -269,6 +269,90 @@ def run_c2(capi, reps=5):
     return {"stream_26q": stream, "hamiltonian_64_words_26q": ham_line, "csr_20q": csr}
 
 
+MFMA_FP32_PEAK_TFLOPS = 157.3
+
+
+def run_c5(capi, n=32, layers=40):
+    """Large-state fused form: the 32-qubit complex64 circuit of 40 layers of
+    Haar SU(2) on every wire and CZ bricks, fused into 5-qubit 32x32 blocks
+    applied on the matrix cores; the same circuit on complex128 through the
+    plain path is the reference (error reported on the TFIM energy, the norm
+    and 4096 sampled amplitudes), and complex64 through the plain path is
+    timed beside it. Per block pass: 2 S bytes and 8 * 32 * 2^n flops."""
+    from helpers import c5_workload, to_capi
+
+    n, ops, ham = c5_workload(n=n, layers=layers)
+    cops = capi.Ops(*ops)
+    h = to_capi(capi, ham)
+    out = {"workload": f"{n} qubits, {layers} layers of Haar SU(2) on every wire + CZ brick, "
+                       f"{len(ops[0])} ops, TFIM observable", "qubits": n, "ops": len(ops[0])}
+    a = capi.State(n, np.complex64)
+    capi.profiling_enable(True)
+    t0 = time.perf_counter()
+    nb = a.apply_ops_blocks(cops)
+    a.synchronize()
+    dt = time.perf_counter() - t0
+    st = _collect(capi)["apply_block5"]
+    k_ms = st["total_ms"] / st["launches"]
+    S = (1 << n) * 8
+    flops = 8.0 * 32 * (1 << n)
+    gbps = 2.0 * S / (k_ms * 1e-3) / 1e9
+    tfl = flops / (k_ms * 1e-3) / 1e12
+    out["blocks_complex64"] = {"ms": round(1e3 * dt, 1), "blocks": nb, "ops_per_block": round(len(ops[0]) / nb, 2),
+                               "kernel_ms_per_block": round(k_ms, 3), "kernel_GBps": round(gbps, 1),
+                               "hbm_frac": round(gbps / HBM_PEAK_GBS, 3), "mfma_TFLOPs": round(tfl, 1),
+                               "mfma_frac": round(tfl / MFMA_FP32_PEAK_TFLOPS, 3)}
+    log(f"bench: C5 blocks {1e3 * dt:.0f} ms, {nb} blocks, {k_ms:.2f} ms/block, {gbps:.0f} GB/s, {tfl:.1f} TFLOP/s")
+    e64 = a.expval(h)
+    idx = np.random.default_rng(32).integers(0, 1 << n, size=4096)
+    amps64 = _amplitudes(capi, a, idx)
+    norm64 = float(np.sqrt(a.probs([0]).sum()))
+    del a
+    b = capi.State(n, np.complex128)
+    t0 = time.perf_counter()
+    b.apply_ops(cops)
+    b.synchronize()
+    dt128 = time.perf_counter() - t0
+    e128 = b.expval(h)
+    amps128 = _amplitudes(capi, b, idx)
+    del b
+    c = capi.State(n, np.complex64)
+    t0 = time.perf_counter()
+    c.apply_ops(cops)
+    c.synchronize()
+    dtp = time.perf_counter() - t0
+    del c
+    out["plain_complex128_ms"] = round(1e3 * dt128, 1)
+    out["plain_complex64_ms"] = round(1e3 * dtp, 1)
+    out["error_vs_complex128"] = {"tfim_energy": abs(e64 - e128), "energy": e128,
+                                  "max_abs_amplitude_4096_samples": float(np.max(np.abs(amps64 - amps128))),
+                                  "norm_minus_1": norm64 - 1.0}
+    log(f"bench: C5 complex128 plain {1e3 * dt128:.0f} ms, complex64 plain {1e3 * dtp:.0f} ms, "
+        f"|dE| = {abs(e64 - e128):.2e}")
+    return out
+
+
+def _amplitudes(capi, s, idx):
+    """Amplitudes at the given indices, copied from the device one by one
+    (a 32-qubit state is 32-64 GiB: it is never downloaded whole)."""
+    import ctypes
+
+    try:
+        hip = ctypes.CDLL("libamdhip64.so")
+    except OSError:
+        hip = ctypes.CDLL("/opt/rocm/lib/libamdhip64.so")
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    s.synchronize()
+    base = s.device_ptr()
+    item = 8 if s.dtype == np.complex64 else 16
+    out = np.zeros(len(idx), dtype=s.dtype)
+    for k, i in enumerate(idx):
+        rc = hip.hipMemcpy(out[k:].ctypes.data, base + int(i) * item, item, 2)  # device to host
+        if rc != 0:
+            raise RuntimeError(f"hipMemcpy failed ({rc})")
+    return out.astype(np.complex128)
+
+
 def copy_ceiling(capi, n=24, reps=10):
     """Device-to-device copies of a complex128 state (hipMemcpyAsync): read +
     written bytes over time, the practical HBM ceiling of a streaming pass."""
@@ -373,6 +457,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--c64-steps", type=int, default=3)
     ap.add_argument("--c2", type=int, default=1, help="run the gate-throughput workload (0 skips it)")
+    ap.add_argument("--c5", type=int, default=1, help="run the 32-qubit fused-block workload (0 skips it)")
     ap.add_argument("--batched", type=int, default=4096,
                     help="circuits of the batched workload (0 skips it)")
     ap.add_argument("--batched-seq", type=int, default=4096,
@@ -456,6 +541,10 @@ def main():
     if args.c2:
         log("bench: gate-throughput workload (26-qubit stream, 64-word <H>, 20-qubit CSR)")
         extra["c2"] = run_c2(capi)
+
+    if args.c5:
+        log("bench: large-state fused form (32 qubits, MFMA blocks)")
+        extra["c5"] = run_c5(capi)
 
     if args.batched > 0:
         log(f"bench: batched workload ({args.batched} circuits)")

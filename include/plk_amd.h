@@ -113,6 +113,14 @@ int plk_apply_matrix(plk_state *sv, const double *matrix, const size_t *wires,
  * HBM passes. Replaces apply(names, wires, adjoints, params)
  * (Bindings.cpp:233-242; StateVectorKokkos.hpp:640-676). */
 int plk_apply_ops(plk_state *sv, const plk_ops *ops);
+/* plk_apply_ops on a complex64 state of at least 9 qubits with runs of
+ * operations fused into dense 32x32 blocks on five wires, each block one
+ * pass of a matrix-core (MFMA) kernel. Replaces the per-gate launches of
+ * apply(names, wires, adjoints, params) (StateVectorKokkos.hpp:640-676) and
+ * the dense multiQubitOpFunctor path (GateFunctors.hpp:195-300) for
+ * circuits of one-to-five-wire gates. *num_blocks (may be NULL) receives
+ * the number of block passes. */
+int plk_apply_ops_blocks(plk_state *sv, const plk_ops *ops, size_t *num_blocks);
 /* Apply the generator G of a parametric gate in place and return the scale s
  * with U(theta) = exp(i s theta G). The adjoint flag is accepted and ignored,
  * as in the reference. Replaces applyGenerator (Bindings.cpp:262-265;

@@ -31,9 +31,9 @@ HIP_FLAGS = [
     "-I", INCLUDE, "-I", CSRC,
 ]
 
-LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "kernels_apply.hip",
+LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "fuse.cpp", "kernels_apply.hip",
                "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip",
-               "kernels_sample.hip"]
+               "kernels_sample.hip", "kernels_block.hip"]
 
 # The register-tile kernel is compiled once per (precision, observable chunk)
 # variant, each as an object of its own (RT_NJMAX in csrc/plk_internal.hpp).

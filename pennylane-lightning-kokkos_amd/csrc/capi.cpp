@@ -2580,6 +2580,33 @@ int plk_apply_ops(plk_state *sv, const plk_ops *ops) {
     })
 }
 
+int plk_apply_ops_blocks(plk_state *sv, const plk_ops *ops, size_t *num_blocks) {
+    PLK_TRY({
+        State &s = S(sv);
+        if (!ops) {
+            raise(PLK_ERR_INVALID, "null ops handle");
+        }
+        if (s.dtype != PLK_C64) {
+            raise(PLK_ERR_UNSUPPORTED, "5-qubit MFMA blocks need a complex64 state");
+        }
+        if (s.n < 9) {
+            raise(PLK_ERR_UNSUPPORTED, "5-qubit MFMA blocks need at least 9 qubits");
+        }
+        const OpsData &d = ops->data;
+        std::vector<CompiledOp> cops;
+        cops.reserve(d.names.size());
+        for (size_t i = 0; i < d.names.size(); i++) {
+            cops.push_back(compile_op(d.names[i], d.wires[i], d.inverses[i], d.params[i],
+                                      d.matrices[i], s.n));
+        }
+        const std::vector<FusedBlock> blocks = fuse_blocks(cops, s.n);
+        apply_blocks(dev_of(s), s, blocks);
+        if (num_blocks) {
+            *num_blocks = blocks.size();
+        }
+    })
+}
+
 int plk_apply_generator(plk_state *sv, const char *name, const size_t *wires,
                         size_t num_wires, int adjoint, double *scale) {
     PLK_TRY({

@@ -462,6 +462,27 @@ double csr_expval_once(Device &dev, const State &s, const CsrView &m);
 // malformed matrices.
 void csr_spmv(Device &dev, const State &out, const State &in, const CsrDevice &m);
 
+// Dense 5-qubit blocks on the matrix cores (kernels_block.hip, fuse.cpp).
+// A fused block: five ascending state bits and a 32x32 complex matrix over
+// the local index s = sum_t bit(bits[t]) << t (row-major).
+struct FusedBlock {
+    int bits[5];
+    std::vector<cd> M;
+    int nops = 0; // operations multiplied into M
+};
+// Greedy fusion of an op list (in order) into blocks of at most five wires:
+// a block starts at the first op not yet fused and takes every later op
+// whose wires fit in five, skipping (and blocking the wires of) ops that do
+// not; ops commute past skipped ops only when their wires are disjoint.
+// Blocks with fewer wires are widened with untouched low bits.
+std::vector<FusedBlock> fuse_blocks(const std::vector<CompiledOp> &ops, int n);
+// The 64 x 64 per-lane operand layout of block5_kernel for M.
+int block5_lane_pack(const std::vector<cd> &M, std::vector<float> &out);
+// One block pass of a complex64 state (n >= 9); d_apack = block5_lane_pack.
+void launch_block5(Device &dev, const State &s, const int *bits, const float *d_apack);
+// Uploads the lane packs of every block (one copy) and applies them in order.
+void apply_blocks(Device &dev, const State &s, const std::vector<FusedBlock> &blocks);
+
 // Probabilities and sampling (kernels_sample.hip). probs: 2^m FP64 values,
 // bin bit (m-1-k) taken from global bit gbits[k]. sample: shots rows of n
 // bits, wire 0 first.

@@ -48,6 +48,7 @@ SIGNATURES = {
                                  _c_double_p, ctypes.c_size_t]),
     "plk_apply_matrix": (ctypes.c_int, [_vp, _c_double_p, _c_size_p, ctypes.c_size_t, ctypes.c_int]),
     "plk_apply_ops": (ctypes.c_int, [_vp, _vp]),
+    "plk_apply_ops_blocks": (ctypes.c_int, [_vp, _vp, _c_size_p]),
     "plk_apply_generator": (ctypes.c_int, [_vp, ctypes.c_char_p, _c_size_p, ctypes.c_size_t,
                                            ctypes.c_int, _c_double_p]),
     "plk_expval_named": (ctypes.c_int, [_vp, ctypes.c_char_p, _c_size_p, ctypes.c_size_t, _c_double_p]),
@@ -239,6 +240,13 @@ class State:
 
     def apply_ops(self, ops: "Ops"):
         check(load().plk_apply_ops(self.h, ops.h))
+
+    def apply_ops_blocks(self, ops: "Ops") -> int:
+        """Apply ops fused into 5-qubit dense blocks on the matrix cores
+        (complex64, >= 9 qubits); returns the number of block passes."""
+        nb = ctypes.c_size_t()
+        check(load().plk_apply_ops_blocks(self.h, ops.h, ctypes.byref(nb)))
+        return nb.value
 
     def apply_generator(self, name: str, wires: Sequence[int], adjoint: bool = False) -> float:
         w = _sizes(wires)

@@ -254,6 +254,42 @@ def c3_workload(n=24, layers=10, seed=20261204, ham_terms=32):
     return n, (names, params, wires, [False] * len(names), [[] for _ in names]), obs
 
 
+def haar_su2(rng):
+    """A Haar-random SU(2) matrix: QR of a complex Gaussian matrix with the
+    phases of R's diagonal moved into Q (Mezzadri 2007), divided by a square
+    root of its determinant."""
+    z = (rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2))) / np.sqrt(2)
+    q, r = np.linalg.qr(z)
+    d = np.diag(r)
+    q = q * (d / np.abs(d))
+    return q / np.sqrt(np.linalg.det(q))
+
+
+def c5_workload(n=32, layers=40, seed=20261207):
+    """Large-state fused form: each layer applies a Haar SU(2) (as a matrix op
+    named QubitUnitary) to every wire, then CZ on a brick pattern (pairs
+    (0,1), (2,3), ... on even layers, (1,2), (3,4), ... on odd layers).
+    Observable: the open-chain transverse-field Ising Hamiltonian
+    sum_i Z_i Z_{i+1} + 0.5 sum_i X_i."""
+    rng = np.random.default_rng(seed)
+    names, params, wires, mats = [], [], [], []
+    for layer in range(layers):
+        for w in range(n):
+            names.append("QubitUnitary")
+            params.append([])
+            wires.append([w])
+            mats.append(haar_su2(rng).reshape(-1))
+        for w in range(layer % 2, n - 1, 2):
+            names.append("CZ")
+            params.append([])
+            wires.append([w, w + 1])
+            mats.append([])
+    coeffs = [1.0] * (n - 1) + [0.5] * n
+    terms = ([("tensor", [("named", "PauliZ", [i]), ("named", "PauliZ", [i + 1])]) for i in range(n - 1)]
+             + [("named", "PauliX", [i]) for i in range(n)])
+    return n, (names, params, wires, [False] * len(names), mats), ("ham", coeffs, terms)
+
+
 def c4_workload(count=4096, seed=20261205, nmin=6, nmax=16, dmin=2, dmax=8):
     """Independent circuits: n uniform in [nmin, nmax], depth uniform in
     [dmin, dmax] layers of RY on every wire, RZ on every wire, then the CNOT
