@@ -269,6 +269,43 @@ def run_c2(capi, reps=5):
     return {"stream_26q": stream, "hamiltonian_64_words_26q": ham_line, "csr_20q": csr}
 
 
+def run_c1(capi, reps=50):
+    """The plain-method workload (12 qubits, 4 layers of RX, RY, RZ and a
+    CNOT ring, 144 parameters, PauliZ on every wire): latency of one full
+    adjoint Jacobian (forward pass included) through plk_adjoint_jacobian,
+    complex128, median over `reps` calls with new angles each call (handles
+    built before the clock, as a training loop would hold them)."""
+    from helpers import c1_workload, to_capi
+
+    n, ops, obs = c1_workload()
+    rng = np.random.default_rng(20261208)
+    s = capi.State(n)
+    cobs = [to_capi(capi, o) for o in obs]
+    npar = sum(1 for p in ops[1] if p)
+    tp = list(range(npar))
+    variants = [capi.Ops(ops[0], [[float(rng.uniform(0, 2 * np.pi))] if p else [] for p in ops[1]],
+                         ops[2], ops[3], ops[4]) for _ in range(reps)]
+    capi.adjoint_jacobian(s, cobs, variants[0], tp, True)
+    times = []
+    capi.profiling_enable(True)
+    for v in variants:
+        s.reset()
+        t0 = time.perf_counter()
+        capi.adjoint_jacobian(s, cobs, v, tp, True)
+        times.append(time.perf_counter() - t0)
+    st = _collect(capi)
+    host = {k: round(v["total_ms"] / v["launches"], 3) for k, v in st.items() if k.startswith("host_adjoint")}
+    med = float(np.median(times))
+    log(f"bench: C1 12q Jacobian {1e3 * med:.3f} ms (median of {reps})")
+    return {"workload": f"{n} qubits, 192 ops, {npar} params, {len(obs)} PauliZ observables, complex128",
+            "jacobian_ms_median": round(1e3 * med, 3), "jacobian_ms_min": round(1e3 * min(times), 3),
+            "params_per_s": round(npar / med, 1), "reps": reps,
+            "host_ms_per_call": host,
+            "note": ("structures seen before run as a replayed HIP graph (capture on the second call); "
+                     "host_adjoint_plan is the host planning of each call, host_adjoint_graph the table "
+                     "upload, replay and result copy")}
+
+
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
@@ -456,6 +493,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-sample", type=int, default=24)
     ap.add_argument("--c64-steps", type=int, default=3)
+    ap.add_argument("--c1", type=int, default=1, help="time the 12-qubit Jacobian latency (0 skips it)")
     ap.add_argument("--c2", type=int, default=1, help="run the gate-throughput workload (0 skips it)")
     ap.add_argument("--c5", type=int, default=1, help="run the 32-qubit fused-block workload (0 skips it)")
     ap.add_argument("--batched", type=int, default=4096,
@@ -537,6 +575,9 @@ def main():
         extra["complex64"] = {"params_per_s": round(ntp * args.c64_steps / dt64, 1),
                               "ms_per_step": round(1e3 * dt64 / args.c64_steps, 2),
                               "max_abs_diff_vs_complex128": float(np.max(np.abs(jac64 - jac)))}
+
+    if args.c1:
+        extra["c1"] = run_c1(capi)
 
     if args.c2:
         log("bench: gate-throughput workload (26-qubit stream, 64-word <H>, 20-qubit CSR)")

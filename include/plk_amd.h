@@ -113,9 +113,10 @@ int plk_apply_matrix(plk_state *sv, const double *matrix, const size_t *wires,
  * HBM passes. Replaces apply(names, wires, adjoints, params)
  * (Bindings.cpp:233-242; StateVectorKokkos.hpp:640-676). */
 int plk_apply_ops(plk_state *sv, const plk_ops *ops);
-/* plk_apply_ops on a complex64 state of at least 9 qubits with runs of
- * operations fused into dense 32x32 blocks on five wires, each block one
- * pass of a matrix-core (MFMA) kernel. Replaces the per-gate launches of
+/* plk_apply_ops on a state of at least 9 qubits with runs of operations
+ * fused into dense 32x32 blocks on five wires, each block one pass of a
+ * matrix-core (MFMA) kernel: FP32 MFMA for complex64 states, FP64 MFMA for
+ * complex128 states. Replaces the per-gate launches of
  * apply(names, wires, adjoints, params) (StateVectorKokkos.hpp:640-676) and
  * the dense multiQubitOpFunctor path (GateFunctors.hpp:195-300) for
  * circuits of one-to-five-wire gates. *num_blocks (may be NULL) receives

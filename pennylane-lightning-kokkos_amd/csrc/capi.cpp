@@ -42,6 +42,10 @@ struct ObsNode {
     // use and kept while the observable lives (entry points hold the engine
     // lock, so the cache needs no lock of its own).
     mutable std::map<std::pair<int, int>, std::unique_ptr<CsrDevice>> csr_dev;
+    // Device Pauli tables per (device, qubit count) when the observable is a
+    // sum of Pauli words (made on first use, kept while the observable lives).
+    mutable std::map<std::pair<int, int>, std::unique_ptr<PauliDevice>> pauli_dev;
+    const PauliDevice *pauli_on(Device &dev, int n) const;
 
     CsrView csr_view() const {
         CsrView v;
@@ -858,55 +862,42 @@ void build_pass(const Pass &ps, int n, std::vector<unsigned char> &table,
 }
 
 // Device tables of segment/pass descriptors for a whole plan (one upload).
-struct SegTable {
-    void *d = nullptr;       // packed op groups
-    void *bundles = nullptr;
-    ~SegTable() {
-        if (d || bundles) {
-            (void)hipDeviceSynchronize();
-        }
-        if (d) {
-            (void)hipFree(d);
-        }
-        if (bundles) {
-            (void)hipFree(bundles);
-        }
-    }
+// Host images of a plan's device tables: op groups of register segments and
+// LDS bundles, the bundle table, and register-tile programs, with the
+// launch metadata that points into them.
+struct RtProgram {
+    RtLaunch L{};
+    size_t off = 0; // byte offset of the program in the plan blob
+};
+
+struct PlanImage {
+    std::vector<unsigned char> ops;
+    std::vector<TileBundle> bundles;
+    std::vector<SegMeta> metas;
+    std::vector<unsigned char> rt;
+    std::vector<RtProgram> progs;
+};
+
+// Device addresses of an image's tables.
+struct PlanPlace {
+    const unsigned char *ops = nullptr;
+    const TileBundle *bundles = nullptr;
+    const unsigned char *rt = nullptr;
 };
 
 template <typename T>
-std::vector<SegMeta> upload_plan(const std::vector<Step> &steps, int n, int device,
-                                 SegTable &tab) {
-    std::vector<unsigned char> table;
-    std::vector<TileBundle> btab;
-    std::vector<SegMeta> metas;
+void build_seg_tables(const std::vector<Step> &steps, int n, PlanImage &img) {
     for (const auto &s : steps) {
         if (s.fused) {
-            metas.push_back(build_segment<T>(s.seg, n, table));
+            img.metas.push_back(build_segment<T>(s.seg, n, img.ops));
         } else if (s.tiled && !s.pass.rt) {
             SegMeta m;
-            build_pass<T>(s.pass, n, table, btab, m);
-            metas.push_back(m);
+            build_pass<T>(s.pass, n, img.ops, img.bundles, m);
+            img.metas.push_back(m);
         } else {
-            metas.push_back(SegMeta{});
+            img.metas.push_back(SegMeta{});
         }
     }
-    if (!table.empty()) {
-        tab.d = dev_malloc(device, table.size(), "operation table");
-        PLK_HIP(hipMemcpy(tab.d, table.data(), table.size(), hipMemcpyHostToDevice));
-    }
-    if (!btab.empty()) {
-        tab.bundles = dev_malloc(device, btab.size() * sizeof(TileBundle), "bundle table");
-        PLK_HIP(hipMemcpy(tab.bundles, btab.data(), btab.size() * sizeof(TileBundle),
-                          hipMemcpyHostToDevice));
-    }
-    return metas;
-}
-
-std::vector<SegMeta> upload_plan_any(const std::vector<Step> &steps, const State &s,
-                                     SegTable &tab) {
-    return s.dtype == PLK_C64 ? upload_plan<float>(steps, s.n, s.device, tab)
-                              : upload_plan<double>(steps, s.n, s.device, tab);
 }
 
 // ---------------------------------------------------------------------------
@@ -923,11 +914,6 @@ std::vector<SegMeta> upload_plan_any(const std::vector<Step> &steps, const State
 // farthest away (Belady). The final layout is the store layout.
 // ---------------------------------------------------------------------------
 int rt_register_bits(int dtype) { return dtype == PLK_C64 ? 4 : 3; }
-
-struct RtProgram {
-    RtLaunch L{};
-    size_t off = 0; // byte offset of the program in the plan blob
-};
 
 template <typename T> struct RtBuilder {
     std::vector<unsigned char> &blob;
@@ -1307,31 +1293,86 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
 }
 
 // Programs of every register-tile pass of a plan, uploaded as one blob.
-struct RtTable {
-    void *d = nullptr;
-    ~RtTable() {
-        if (d) {
-            (void)hipDeviceSynchronize();
-            (void)hipFree(d);
-        }
+PlanImage build_image(const std::vector<Step> &steps, const State &s) {
+    PlanImage img;
+    if (s.dtype == PLK_C64) {
+        build_seg_tables<float>(steps, s.n, img);
+    } else {
+        build_seg_tables<double>(steps, s.n, img);
     }
-};
-
-std::vector<RtProgram> upload_rt_plan(const std::vector<Step> &steps, const State &s, RtTable &tab) {
-    std::vector<unsigned char> blob;
-    std::vector<RtProgram> progs(steps.size());
+    img.progs.resize(steps.size());
     const int R = rt_register_bits(s.dtype);
     for (size_t i = 0; i < steps.size(); i++) {
         if (steps[i].tiled && steps[i].pass.rt) {
-            progs[i] = s.dtype == PLK_C64 ? build_rt_program<float>(steps[i].pass, s.n, R, blob)
-                                          : build_rt_program<double>(steps[i].pass, s.n, R, blob);
+            img.progs[i] = s.dtype == PLK_C64 ? build_rt_program<float>(steps[i].pass, s.n, R, img.rt)
+                                              : build_rt_program<double>(steps[i].pass, s.n, R, img.rt);
         }
     }
-    if (!blob.empty()) {
-        tab.d = dev_malloc(s.device, blob.size(), "register-tile programs");
-        PLK_HIP(hipMemcpy(tab.d, blob.data(), blob.size(), hipMemcpyHostToDevice));
+    return img;
+}
+
+size_t image_bytes(const PlanImage &img) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+    return up(img.ops.size()) + up(img.bundles.size() * sizeof(TileBundle)) + up(img.rt.size());
+}
+
+// Copies the image's tables to base (image_bytes(img) bytes) on the device's
+// stream; the image must stay alive until the stream has run the copies.
+PlanPlace copy_image(Device &dev, const PlanImage &img, unsigned char *base) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+    PlanPlace pl;
+    size_t off = 0;
+    auto put = [&](const void *src, size_t bytes) -> unsigned char * {
+        unsigned char *dst = base + off;
+        if (bytes) {
+            PLK_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, dev.stream));
+        }
+        off += up(bytes);
+        return dst;
+    };
+    pl.ops = put(img.ops.data(), img.ops.size());
+    pl.bundles = reinterpret_cast<const TileBundle *>(
+        put(img.bundles.data(), img.bundles.size() * sizeof(TileBundle)));
+    pl.rt = put(img.rt.data(), img.rt.size());
+    return pl;
+}
+
+// The addresses copy_image(dev, img, base) fills, without copying.
+PlanPlace place_addresses(const PlanImage &img, unsigned char *base) {
+    auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
+    PlanPlace pl;
+    pl.ops = base;
+    pl.bundles = reinterpret_cast<const TileBundle *>(base + up(img.ops.size()));
+    pl.rt = base + up(img.ops.size()) + up(img.bundles.size() * sizeof(TileBundle));
+    return pl;
+}
+
+PlanPlace place_in_arena(Device &dev, const PlanImage &img) {
+    const size_t bytes = image_bytes(img);
+    return copy_image(dev, img, static_cast<unsigned char *>(arena_alloc(dev, bytes ? bytes : 256)));
+}
+
+// Everything a replay of the plan's launches takes as kernel arguments:
+// equal signatures mean equal launches, whatever the matrices in the tables.
+void append_signature(const PlanImage &img, std::vector<unsigned char> &sig) {
+    auto put = [&](const void *p, size_t bytes) {
+        const auto *b = static_cast<const unsigned char *>(p);
+        sig.insert(sig.end(), b, b + bytes);
+    };
+    const uint64_t sizes[3] = {img.ops.size(), img.bundles.size(), img.rt.size()};
+    put(sizes, sizeof sizes);
+    for (const SegMeta &m : img.metas) {
+        put(&m.L, sizeof m.L);
+        put(&m.first, sizeof m.first);
+        put(m.cols, sizeof m.cols);
+        put(m.coefs, sizeof m.coefs);
+        put(&m.TL, sizeof m.TL);
+        put(&m.first_bundle, sizeof m.first_bundle);
     }
-    return progs;
+    for (const RtProgram &p : img.progs) {
+        put(&p.L, sizeof p.L);
+        put(&p.off, sizeof p.off);
+    }
 }
 
 // Observable states per register-tile launch: 3 keeps phi plus three states
@@ -1342,8 +1383,12 @@ int rtile_chunk(int dtype) {
     return std::max(1, std::min(RT_NJMAX, env_int("PLK_RTILE_NJ", 3)));
 }
 
-// Register-tile passes serve states with at least as many tiles as CUs (and
-// PLK_RTILE=0 turns them off, PLK_RTILE=1 forces them from the tile width up).
+// Register-tile passes serve small states, where their few launches set the
+// latency, and states with at least as many tiles as CUs; between the two
+// the LDS tile passes measured faster (24-layer adjoint at 16 qubits
+// complex128: 2.12 vs 2.36 ms; at 10 and 12 qubits register tiles take 1.42
+// and 1.71 ms against 2.49 and 3.09). PLK_RTILE=0 turns them off,
+// PLK_RTILE=1 forces them from the tile width up.
 bool use_rtile(int n, int dtype) {
     const int T = rt_register_bits(dtype) + RT_LANE_BITS;
     const int mode = env_int("PLK_RTILE", -1);
@@ -1353,42 +1398,46 @@ bool use_rtile(int n, int dtype) {
     if (mode == 1) {
         return n >= T;
     }
-    return n >= T + 8;
+    return n >= T && (n <= T + 5 || n >= T + 8);
 }
 
 void apply_single(Device &dev, const State &s, const CompiledOp &op);
 
 // Applies a forward plan (no observables) to s.
-void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
-    SegTable tab;
-    RtTable rtab;
-    auto metas = upload_plan_any(steps, s, tab);
-    const auto progs = upload_rt_plan(steps, s, rtab);
+// Launches a forward plan whose tables are at pl (no host synchronisation
+// unless a step is a single op outside the fused kernels).
+void enqueue_forward(Device &dev, const State &s, const std::vector<Step> &steps,
+                     const PlanImage &img, const PlanPlace &pl) {
     for (size_t i = 0; i < steps.size(); i++) {
         if (steps[i].tiled && steps[i].pass.rt) {
-            RtLaunch L = progs[i].L;
+            RtLaunch L = img.progs[i].L;
             L.nj = 0;
             L.write_phi = 1;
             L.nslots = 0;
-            launch_rtile(dev, s, nullptr, static_cast<const char *>(rtab.d) + progs[i].off, L);
+            launch_rtile(dev, s, nullptr, pl.rt + img.progs[i].off, L);
         } else if (steps[i].fused) {
-            SegLaunch L = metas[i].L;
+            SegLaunch L = img.metas[i].L;
             L.write_phi = 1;
             L.nj = 0;
             L.nslots = 0;
-            const void *ops = static_cast<const char *>(tab.d) + metas[i].first;
-            launch_segment(dev, s, nullptr, ops, L);
+            launch_segment(dev, s, nullptr, pl.ops + img.metas[i].first, L);
         } else if (steps[i].tiled) {
-            TileLaunch TL = metas[i].TL;
+            TileLaunch TL = img.metas[i].TL;
             TL.nj = 0;
             TL.write_phi = 1;
             TL.nslots = 0;
-            const void *b = static_cast<const TileBundle *>(tab.bundles) + metas[i].first_bundle;
-            launch_tile(dev, s, nullptr, b, tab.d, TL);
+            launch_tile(dev, s, nullptr, pl.bundles + img.metas[i].first_bundle, pl.ops, TL);
         } else {
             apply_single(dev, s, steps[i].single.U);
         }
     }
+}
+
+void run_forward(Device &dev, const State &s, const std::vector<Step> &steps) {
+    const PlanImage img = build_image(steps, s);
+    enqueue_forward(dev, s, steps, img, place_in_arena(dev, img));
+    // The host image is read by the stream's copies.
+    PLK_HIP(hipStreamSynchronize(dev.stream));
 }
 
 void apply_single(Device &dev, const State &s, const CompiledOp &op) {
@@ -1425,6 +1474,20 @@ void apply_generator_dev(Device &dev, const State &s, const CompiledGen &g) {
 // ---------------------------------------------------------------------------
 void apply_obs_general(Device &dev, const State &s, const ObsNode &o);
 
+const PauliDevice *ObsNode::pauli_on(Device &dev, int n) const {
+    auto it = pauli_dev.find({dev.id, n});
+    if (it != pauli_dev.end()) {
+        return it->second.get();
+    }
+    std::vector<PauliTerm> terms;
+    if (!to_pauli(*this, n, terms)) {
+        return nullptr;
+    }
+    auto &slot = pauli_dev[{dev.id, n}];
+    slot = pauli_upload(dev, terms);
+    return slot.get();
+}
+
 void check_sparse_wires(const ObsNode &o, const State &s) {
     if (o.wires.size() != static_cast<size_t>(s.n)) {
         raise(PLK_ERR_INVALID, "SparseH wire count does not match state-vector size");
@@ -1432,9 +1495,8 @@ void check_sparse_wires(const ObsNode &o, const State &s) {
 }
 
 void apply_obs_into(Device &dev, const State &out, const State &in, const ObsNode &o) {
-    std::vector<PauliTerm> terms;
-    if (to_pauli(o, in.n, terms)) {
-        launch_pauli_apply(dev, out, in, terms);
+    if (const PauliDevice *t = o.pauli_on(dev, in.n)) {
+        launch_pauli_apply(dev, out, in, *t);
         return;
     }
     if (o.kind == ObsNode::Sparse) {
@@ -1488,9 +1550,8 @@ void apply_obs_general(Device &dev, const State &s, const ObsNode &o) {
 }
 
 double expval_obs(Device &dev, const State &s, const ObsNode &o) {
-    std::vector<PauliTerm> terms;
-    if (to_pauli(o, s.n, terms)) {
-        return pauli_expval(dev, s, terms);
+    if (const PauliDevice *t = o.pauli_on(dev, s.n)) {
+        return pauli_expval(dev, s, *t);
     }
     if (o.kind == ObsNode::Sparse) {
         // sum_r Re(conj(psi_r) (A psi)_r) without materialising A psi.
@@ -1588,15 +1649,217 @@ std::vector<PlanOp> reverse_ops(const OpsData &ops, const std::vector<size_t> &t
     return rev;
 }
 
+// 64-bit structure hash (multiply-xorshift over 64-bit words).
+struct Hasher {
+    uint64_t h = 0x243F6A8885A308D3ull;
+    void add(uint64_t v) {
+        h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
+        h *= 0xFF51AFD7ED558CCDull;
+        h ^= h >> 32;
+    }
+    void add(double v) {
+        uint64_t b;
+        std::memcpy(&b, &v, sizeof b);
+        add(b);
+    }
+    void add(const std::string &str) {
+        add(static_cast<uint64_t>(str.size()));
+        add(static_cast<uint64_t>(std::hash<std::string>{}(str)));
+    }
+};
+
 // ---------------------------------------------------------------------------
 // Adjoint Jacobian (AdjointDiffKokkos.hpp:404-478).
 //
 // The reverse walk over the operations, the trainable-parameter bookkeeping
 // and the early stop follow the reference line by line; the per-operation
 // work (mu copy, U^dagger on lambda, generator on mu, n_obs inner products,
-// U^dagger on every H lambda) is replaced by fused segments.
+// U^dagger on every H lambda) is replaced by fused passes.
 // ---------------------------------------------------------------------------
-void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
+
+// Launches reverse step i (a fused segment, register-tile or LDS-tile pass
+// over lambda and every H_j lambda, with its Jacobian reductions); returns
+// false for a single op, which the caller runs.
+bool enqueue_reverse_step(Device &dev, const State &lambda, void *const *hptr, size_t n_obs,
+                          size_t n_tp, const Step &st, const PlanImage &img, size_t i,
+                          const PlanPlace &pl, double *d_jac) {
+    if (st.fused) {
+        const SegMeta &m = img.metas[i];
+        const int njmax = std::min(SEG_NJMAX, seg_max_nj(lambda.dtype, m.L.K));
+        size_t j0 = 0;
+        while (j0 < n_obs) {
+            const size_t left = n_obs - j0;
+            int nj = 1;
+            for (int c : {8, 4, 2, 1}) {
+                if (c <= njmax && static_cast<size_t>(c) <= left) {
+                    nj = c;
+                    break;
+                }
+            }
+            SegLaunch L = m.L;
+            L.nj = nj;
+            L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+            const int grid = launch_segment(dev, lambda, hptr + j0, pl.ops + m.first, L);
+            if (L.nslots > 0) {
+                launch_segment_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0), m.cols,
+                                      m.coefs, d_jac, static_cast<int>(n_tp));
+            }
+            j0 += nj;
+        }
+        return true;
+    }
+    if (st.tiled && st.pass.rt) {
+        size_t j0 = 0;
+        for (int nj : chunk_sizes(n_obs, rtile_chunk(lambda.dtype))) {
+            RtLaunch L = img.progs[i].L;
+            L.nj = nj;
+            L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+            const int grid = launch_rtile(dev, lambda, hptr + j0, pl.rt + img.progs[i].off, L);
+            if (L.nslots > 0) {
+                launch_tile_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0),
+                                   st.pass.cols.data(), st.pass.coefs.data(), d_jac,
+                                   static_cast<int>(n_tp));
+            }
+            j0 += nj;
+        }
+        return true;
+    }
+    if (st.tiled) {
+        const SegMeta &m = img.metas[i];
+        size_t j0 = 0;
+        for (int nj : chunk_sizes(n_obs, tile_max_nj(lambda.dtype, m.TL.T))) {
+            TileLaunch TL = m.TL;
+            TL.nj = nj;
+            TL.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+            const int grid = launch_tile(dev, lambda, hptr + j0, pl.bundles + m.first_bundle,
+                                         pl.ops, TL);
+            if (TL.nslots > 0) {
+                launch_tile_reduce(dev, grid, TL.nslots, nj, static_cast<int>(j0),
+                                   st.pass.cols.data(), st.pass.coefs.data(), d_jac,
+                                   static_cast<int>(n_tp));
+            }
+            j0 += nj;
+        }
+        return true;
+    }
+    return false;
+}
+
+// A captured sweep: copy of the input state, forward passes, observables,
+// Jacobian zero fill and reverse passes of one circuit structure, replayed
+// with a fresh upload of the plan and Pauli tables (angles and coefficients
+// live there). The graph owns every buffer its kernels touch except the
+// input state and the device's partials buffer (checked before each replay).
+struct SweepGraph {
+    uint64_t key = 0;
+    int device = -1;
+    hipStream_t stream = nullptr;
+    const void *ref = nullptr;
+    std::vector<unsigned char> sig;
+    std::unique_ptr<TmpState> lambda;
+    std::vector<std::unique_ptr<TmpState>> H;
+    std::vector<void *> hptr;
+    void *tables = nullptr;
+    size_t fwd_bytes = 0, rev_bytes = 0;
+    double *jac = nullptr;
+    double *red = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~SweepGraph() {
+        (void)hipSetDevice(device);
+        (void)hipStreamSynchronize(stream);
+        if (exec) {
+            (void)hipGraphExecDestroy(exec);
+        }
+        if (tables) {
+            (void)hipFree(tables);
+        }
+        if (jac) {
+            (void)hipFree(jac);
+        }
+    }
+};
+
+constexpr int kGraphMaxQubits = 18;
+constexpr size_t kGraphsPerDevice = 128;
+constexpr size_t kGraphBytesPerDevice = size_t{4} << 30; // states held by graphs
+
+std::map<int, std::vector<std::unique_ptr<SweepGraph>>> &sweep_graphs() {
+    static std::map<int, std::vector<std::unique_ptr<SweepGraph>>> g; // released with the process
+    return g;
+}
+
+// Structures seen once (a capture pays off from the second call on).
+std::map<int, std::map<uint64_t, int>> &sweep_seen() {
+    static std::map<int, std::map<uint64_t, int>> seen;
+    return seen;
+}
+
+size_t graph_bytes(const SweepGraph &g) {
+    return (1 + g.H.size()) * g.lambda->s.bytes();
+}
+
+// Everything but the angles: op names / wires / inverse flags / parameter
+// counts / matrix sizes, the trainable set, sizes and flags.
+uint64_t sweep_key(const State &ref, const OpsData &ops, const std::vector<size_t> &tp,
+                   size_t n_obs, bool apply_ops) {
+    Hasher H;
+    H.add(static_cast<uint64_t>(ref.n));
+    H.add(static_cast<uint64_t>(ref.dtype));
+    H.add(static_cast<uint64_t>(n_obs));
+    H.add(static_cast<uint64_t>(apply_ops ? 1 : 0));
+    H.add(static_cast<uint64_t>(ops.names.size()));
+    for (size_t i = 0; i < ops.names.size(); i++) {
+        H.add(ops.names[i]);
+        H.add(static_cast<uint64_t>(ops.wires[i].size() | (ops.inverses[i] ? size_t{1} << 40 : 0)));
+        for (auto w : ops.wires[i]) {
+            H.add(static_cast<uint64_t>(w));
+        }
+        H.add(static_cast<uint64_t>(ops.params[i].size()));
+        H.add(static_cast<uint64_t>(ops.matrices[i].size()));
+    }
+    H.add(static_cast<uint64_t>(tp.size()));
+    for (auto t : tp) {
+        H.add(static_cast<uint64_t>(t));
+    }
+    return H.h;
+}
+
+// Stream capture with profiling scopes off; always ends the capture.
+struct CaptureGuard {
+    hipStream_t stream;
+    bool open = true;
+    explicit CaptureGuard(hipStream_t s) : stream(s) {
+        set_capturing(true);
+        PLK_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeRelaxed));
+    }
+    hipGraph_t end() {
+        hipGraph_t g = nullptr;
+        open = false;
+        set_capturing(false);
+        PLK_HIP(hipStreamEndCapture(stream, &g));
+        return g;
+    }
+    ~CaptureGuard() {
+        if (open) {
+            hipGraph_t g = nullptr;
+            set_capturing(false);
+            if (hipStreamEndCapture(stream, &g) == hipSuccess && g) {
+                (void)hipGraphDestroy(g);
+            }
+        }
+    }
+};
+
+bool steps_have_single(const std::vector<Step> &steps) {
+    for (const auto &st : steps) {
+        if (!st.fused && !st.tiled) {
+            return true;
+        }
+    }
+    return false;
+}
+
+void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const ObsNode>> &obs,
                       const OpsData &ops, const std::vector<size_t> &tp, bool apply_ops,
                       double *jac) {
     if (tp.empty()) {
@@ -1609,6 +1872,7 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     const int n = ref.n;
 
     // Reverse plan first: it validates the op list before any device work.
+    std::unique_ptr<HostScope> plan_scope = std::make_unique<HostScope>("host_adjoint_plan");
     std::vector<PlanOp> rev = reverse_ops(ops, tp, n);
     if (n_obs == 0) {
         return;
@@ -1620,11 +1884,151 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
                         rt_register_bits(ref.dtype) + 2)
            : T_adj > 0 ? plan_tiles(std::move(rev), n, T_adj, low_bits(ref.dtype))
                        : plan_steps(std::move(rev));
+    const std::vector<Step> fwd = apply_ops ? forward_plan(ops, n, ref.dtype) : std::vector<Step>{};
+    const PlanImage img_f = build_image(fwd, ref);
+    const PlanImage img_r = build_image(steps, ref);
+    plan_scope.reset();
+
+    // Small circuits run as a replayed HIP graph: their sweeps are hundreds
+    // of launches of a few microseconds each.
+    // Observables enter the graph as Pauli tables copied with the plan
+    // tables, so circuits that differ only in angles or in observable
+    // coefficients share one graph.
+    struct PauliImg {
+        std::vector<unsigned char> bytes;
+        size_t toff = 0;
+        int ngroups = 0;
+    };
+    std::vector<PauliImg> pimg;
+    bool graph = n <= kGraphMaxQubits && env_int("PLK_GRAPH", 1) != 0 &&
+                 !steps_have_single(steps) && !steps_have_single(fwd);
+    for (size_t j = 0; graph && j < n_obs; j++) {
+        std::vector<PauliTerm> terms;
+        graph = to_pauli(*obs[j], n, terms);
+        PauliImg pi;
+        if (graph) {
+            pi.bytes = pauli_image(terms, pi.toff, pi.ngroups);
+        }
+        pimg.push_back(std::move(pi));
+    }
+    const uint64_t key = graph ? sweep_key(ref, ops, tp, n_obs, apply_ops) : 0;
+    if (graph) {
+        auto &seen = sweep_seen()[dev.id];
+        if (seen.size() > 65536) {
+            seen.clear();
+        }
+        graph = ++seen[key] >= 2;
+    }
+    if (graph) {
+        std::unique_ptr<HostScope> phase = std::make_unique<HostScope>("host_adjoint_graph");
+        std::vector<unsigned char> sig;
+        append_signature(img_f, sig);
+        append_signature(img_r, sig);
+        size_t pauli_bytes = 0;
+        for (const auto &pi : pimg) {
+            const uint64_t f[3] = {pi.bytes.size(), pi.toff, static_cast<uint64_t>(pi.ngroups)};
+            const auto *b = reinterpret_cast<const unsigned char *>(f);
+            sig.insert(sig.end(), b, b + sizeof f);
+            pauli_bytes += (pi.bytes.size() + 255) & ~size_t{255};
+        }
+        auto &list = sweep_graphs()[dev.id];
+        SweepGraph *g = nullptr;
+        for (auto it = list.begin(); it != list.end(); ++it) {
+            SweepGraph &c = **it;
+            const bool same = c.key == key && c.ref == ref.d;
+            if (same) {
+                if (c.sig != sig || c.red != dev.red) {
+                    list.erase(it); // stale layout or partials buffer: capture again
+                } else {
+                    std::rotate(it, it + 1, list.end()); // most recently used last
+                    g = list.back().get();
+                }
+                break;
+            }
+        }
+        if (g == nullptr) {
+            // Least recently used first in the list: evict from the front.
+            size_t held = 0;
+            for (const auto &c : list) {
+                held += graph_bytes(*c);
+            }
+            const size_t need = (1 + n_obs) * ref.bytes();
+            while (!list.empty() &&
+                   (list.size() >= kGraphsPerDevice || held + need > kGraphBytesPerDevice)) {
+                held -= graph_bytes(*list.front());
+                list.erase(list.begin());
+            }
+            auto ng = std::make_unique<SweepGraph>();
+            ng->key = key;
+            ng->device = dev.id;
+            ng->stream = dev.stream;
+            ng->ref = ref.d;
+            ng->sig = sig;
+            ng->lambda = std::make_unique<TmpState>(n, ref.dtype, ref.device);
+            for (size_t j = 0; j < n_obs; j++) {
+                ng->H.push_back(std::make_unique<TmpState>(n, ref.dtype, ref.device));
+                ng->hptr.push_back(ng->H.back()->s.d);
+            }
+            ng->fwd_bytes = image_bytes(img_f);
+            ng->rev_bytes = image_bytes(img_r);
+            ng->tables = dev_malloc(dev.id, ng->fwd_bytes + ng->rev_bytes + pauli_bytes + 256,
+                                    "graph tables");
+            ng->jac = static_cast<double *>(dev_malloc(dev.id, sizeof(double) * n_obs * n_tp, "graph Jacobian"));
+            // The launchers grow the partials buffer on demand, which would
+            // synchronise inside the capture: size it for any pass first.
+            ensure_red(dev, static_cast<size_t>(dev.num_cus) * 8 * std::max(TILE_PMAX, RT_PMAX) * 8,
+                       256);
+            auto *tb = static_cast<unsigned char *>(ng->tables);
+            // Addresses only: the tables are copied before every replay.
+            const PlanPlace pf = place_addresses(img_f, tb);
+            const PlanPlace pr = place_addresses(img_r, tb + ng->fwd_bytes);
+            {
+                CaptureGuard cap(dev.stream);
+                const State &lam = ng->lambda->s;
+                launch_copy(dev, lam, ref);
+                enqueue_forward(dev, lam, fwd, img_f, pf);
+                unsigned char *pt = tb + ng->fwd_bytes + ng->rev_bytes;
+                for (size_t j = 0; j < n_obs; j++) {
+                    launch_pauli_apply_at(dev, ng->H[j]->s, lam, pt, pimg[j].toff, pimg[j].ngroups);
+                    pt += (pimg[j].bytes.size() + 255) & ~size_t{255};
+                }
+                launch_fill_zero_doubles(dev, ng->jac, n_obs * n_tp);
+                for (size_t i = 0; i < steps.size(); i++) {
+                    enqueue_reverse_step(dev, lam, ng->hptr.data(), n_obs, n_tp, steps[i], img_r,
+                                         i, pr, ng->jac);
+                }
+                hipGraph_t graph_def = cap.end();
+                const hipError_t e = hipGraphInstantiate(&ng->exec, graph_def, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph_def);
+                PLK_HIP(e);
+            }
+            ng->red = dev.red;
+            list.push_back(std::move(ng));
+            g = list.back().get();
+        }
+        auto *tb = static_cast<unsigned char *>(g->tables);
+        copy_image(dev, img_f, tb);
+        copy_image(dev, img_r, tb + g->fwd_bytes);
+        unsigned char *pt = tb + g->fwd_bytes + g->rev_bytes;
+        for (const auto &pi : pimg) {
+            if (!pi.bytes.empty()) {
+                PLK_HIP(hipMemcpyAsync(pt, pi.bytes.data(), pi.bytes.size(), hipMemcpyHostToDevice,
+                                       dev.stream));
+            }
+            pt += (pi.bytes.size() + 255) & ~size_t{255};
+        }
+        PLK_HIP(hipGraphLaunch(g->exec, dev.stream));
+        PLK_HIP(hipMemcpyAsync(jac, g->jac, sizeof(double) * n_obs * n_tp, hipMemcpyDeviceToHost,
+                               dev.stream));
+        // Also keeps the host images alive until their copies have run.
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+        return;
+    }
 
     TmpState lambda(n, ref.dtype, ref.device);
     launch_copy(dev, lambda.s, ref);
     if (apply_ops) {
-        run_forward(dev, lambda.s, forward_plan(ops, n, ref.dtype));
+        enqueue_forward(dev, lambda.s, fwd, img_f, place_in_arena(dev, img_f));
     }
     std::vector<std::unique_ptr<TmpState>> H;
     for (size_t j = 0; j < n_obs; j++) {
@@ -1642,93 +2046,37 @@ void adjoint_jacobian(const State &ref, const std::vector<const ObsNode *> &obs,
     launch_fill_zero_doubles(dev, d_jac, n_obs * n_tp);
     std::vector<std::pair<size_t, double>> host_entries; // (flat index, value)
 
-    SegTable tab;
-    RtTable rtab;
-    const auto metas = upload_plan_any(steps, lambda.s, tab);
-    const auto progs = upload_rt_plan(steps, lambda.s, rtab);
+    const PlanPlace pl = place_in_arena(dev, img_r);
     std::unique_ptr<TmpState> mu;
-    const int kmax_nj = SEG_NJMAX;
     for (size_t i = 0; i < steps.size(); i++) {
         const Step &st = steps[i];
-        if (st.fused) {
-            const SegMeta &m = metas[i];
-            const void *d_ops = static_cast<const char *>(tab.d) + m.first;
-            const int njmax = std::min(kmax_nj, seg_max_nj(ref.dtype, m.L.K));
-            size_t j0 = 0;
-            while (j0 < n_obs) {
-                const size_t left = n_obs - j0;
-                int nj = 1;
-                for (int c : {8, 4, 2, 1}) {
-                    if (c <= njmax && static_cast<size_t>(c) <= left) {
-                        nj = c;
-                        break;
-                    }
-                }
-                SegLaunch L = m.L;
-                L.nj = nj;
-                L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
-                const int grid = launch_segment(dev, lambda.s, hptr.data() + j0, d_ops, L);
-                if (L.nslots > 0) {
-                    launch_segment_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0), m.cols,
-                                          m.coefs, d_jac, static_cast<int>(n_tp));
-                }
-                j0 += nj;
+        if (enqueue_reverse_step(dev, lambda.s, hptr.data(), n_obs, n_tp, st, img_r, i, pl, d_jac)) {
+            continue;
+        }
+        const PlanOp &p = st.single;
+        if (p.trainable) {
+            if (!mu) {
+                mu = std::make_unique<TmpState>(n, ref.dtype, ref.device);
             }
-        } else if (st.tiled && st.pass.rt) {
-            const void *prog = static_cast<const char *>(rtab.d) + progs[i].off;
-            size_t j0 = 0;
-            for (int nj : chunk_sizes(n_obs, rtile_chunk(ref.dtype))) {
-                RtLaunch L = progs[i].L;
-                L.nj = nj;
-                L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
-                const int grid = launch_rtile(dev, lambda.s, hptr.data() + j0, prog, L);
-                if (L.nslots > 0) {
-                    launch_tile_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0),
-                                       st.pass.cols.data(), st.pass.coefs.data(), d_jac,
-                                       static_cast<int>(n_tp));
-                }
-                j0 += nj;
-            }
-        } else if (st.tiled) {
-            const SegMeta &m = metas[i];
-            const void *b = static_cast<const TileBundle *>(tab.bundles) + m.first_bundle;
-            size_t j0 = 0;
-            for (int nj : chunk_sizes(n_obs, tile_max_nj(ref.dtype, m.TL.T))) {
-                TileLaunch TL = m.TL;
-                TL.nj = nj;
-                TL.write_phi = (j0 + nj == n_obs) ? 1 : 0;
-                const int grid = launch_tile(dev, lambda.s, hptr.data() + j0, b, tab.d, TL);
-                if (TL.nslots > 0) {
-                    launch_tile_reduce(dev, grid, TL.nslots, nj, static_cast<int>(j0),
-                                       st.pass.cols.data(), st.pass.coefs.data(), d_jac, static_cast<int>(n_tp));
-                }
-                j0 += nj;
-            }
-        } else {
-            const PlanOp &p = st.single;
-            if (p.trainable) {
-                if (!mu) {
-                    mu = std::make_unique<TmpState>(n, ref.dtype, ref.device);
-                }
-                launch_copy(dev, mu->s, lambda.s);
-            }
-            apply_single(dev, lambda.s, p.U);
-            if (p.trainable) {
-                apply_generator_dev(dev, mu->s, p.G);
-                std::vector<double> ri(2 * n_obs);
-                inner_products(dev, hstate.data(), static_cast<int>(n_obs), mu->s, ri.data());
-                for (size_t j = 0; j < n_obs; j++) {
-                    host_entries.push_back({j * n_tp + static_cast<size_t>(p.col),
-                                            p.coef * ri[2 * j + 1]});
-                }
-            }
+            launch_copy(dev, mu->s, lambda.s);
+        }
+        apply_single(dev, lambda.s, p.U);
+        if (p.trainable) {
+            apply_generator_dev(dev, mu->s, p.G);
+            std::vector<double> ri(2 * n_obs);
+            inner_products(dev, hstate.data(), static_cast<int>(n_obs), mu->s, ri.data());
             for (size_t j = 0; j < n_obs; j++) {
-                apply_single(dev, H[j]->s, p.U);
+                host_entries.push_back({j * n_tp + static_cast<size_t>(p.col),
+                                        p.coef * ri[2 * j + 1]});
             }
+        }
+        for (size_t j = 0; j < n_obs; j++) {
+            apply_single(dev, H[j]->s, p.U);
         }
     }
     PLK_HIP(hipMemcpyAsync(jac, d_jac, sizeof(double) * n_obs * n_tp, hipMemcpyDeviceToHost,
                            dev.stream));
+    // Also keeps the host images alive until their copies have run.
     PLK_HIP(hipStreamSynchronize(dev.stream));
     for (auto &e : host_entries) {
         jac[e.first] = e.second;
@@ -1989,25 +2337,6 @@ size_t plan_threads(size_t count) {
         {static_cast<size_t>(std::max(1, env_int("PLK_PLAN_THREADS", static_cast<int>(hw)))),
          (count + 31) / 32, size_t{64}});
 }
-
-// 64-bit structure hash (multiply-xorshift over 64-bit words).
-struct Hasher {
-    uint64_t h = 0x243F6A8885A308D3ull;
-    void add(uint64_t v) {
-        h ^= v + 0x9E3779B97F4A7C15ull + (h << 6) + (h >> 2);
-        h *= 0xFF51AFD7ED558CCDull;
-        h ^= h >> 32;
-    }
-    void add(double v) {
-        uint64_t b;
-        std::memcpy(&b, &v, sizeof b);
-        add(b);
-    }
-    void add(const std::string &str) {
-        add(static_cast<uint64_t>(str.size()));
-        add(static_cast<uint64_t>(std::hash<std::string>{}(str)));
-    }
-};
 
 void hash_obs(Hasher &H, const ObsNode &o) {
     H.add(static_cast<uint64_t>(o.kind));
@@ -2357,6 +2686,7 @@ using namespace plk;
 #define PLK_TRY(body)                                                          \
     try {                                                                      \
         std::lock_guard<std::recursive_mutex> plk_engine_lock_(engine_mutex()); \
+        begin_call();                                                          \
         body;                                                                  \
         g_last_error.clear();                                                  \
         return PLK_OK;                                                         \
@@ -2585,9 +2915,6 @@ int plk_apply_ops_blocks(plk_state *sv, const plk_ops *ops, size_t *num_blocks) 
         State &s = S(sv);
         if (!ops) {
             raise(PLK_ERR_INVALID, "null ops handle");
-        }
-        if (s.dtype != PLK_C64) {
-            raise(PLK_ERR_UNSUPPORTED, "5-qubit MFMA blocks need a complex64 state");
         }
         if (s.n < 9) {
             raise(PLK_ERR_UNSUPPORTED, "5-qubit MFMA blocks need at least 9 qubits");
@@ -2821,10 +3148,9 @@ int plk_obs_apply(plk_state *sv, const plk_obs *ob) {
     PLK_TRY({
         State &s = S(sv);
         Device &dev = dev_of(s);
-        std::vector<PauliTerm> terms;
-        if (to_pauli(O(ob), s.n, terms)) {
+        if (const PauliDevice *t = O(ob).pauli_on(dev, s.n)) {
             TmpState tmp(s.n, s.dtype, s.device);
-            launch_pauli_apply(dev, tmp.s, s, terms);
+            launch_pauli_apply(dev, tmp.s, s, *t);
             launch_copy(dev, s, tmp.s);
             PLK_HIP(hipStreamSynchronize(dev.stream));
         } else {
@@ -2872,9 +3198,10 @@ int plk_adjoint_jacobian(const plk_state *sv, const plk_obs *const *obs, size_t 
         if (!ops) {
             raise(PLK_ERR_INVALID, "null ops handle");
         }
-        std::vector<const ObsNode *> ob;
+        std::vector<std::shared_ptr<const ObsNode>> ob;
         for (size_t j = 0; j < num_obs; j++) {
-            ob.push_back(&O(obs[j]));
+            O(obs[j]);
+            ob.push_back(obs[j]->node);
         }
         std::vector<size_t> tp(trainable, trainable + num_trainable);
         adjoint_jacobian(s, ob, ops->data, tp, apply_operations != 0, jac);

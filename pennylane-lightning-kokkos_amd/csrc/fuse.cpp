@@ -162,24 +162,37 @@ std::vector<FusedBlock> fuse_blocks(const std::vector<CompiledOp> &ops_in, int n
     return blocks;
 }
 
-void apply_blocks(Device &dev, const State &s, const std::vector<FusedBlock> &blocks) {
-    if (blocks.empty()) {
-        return;
-    }
-    std::vector<float> all(blocks.size() * 4096);
-    std::vector<float> one;
+namespace {
+
+template <typename T>
+void apply_blocks_t(Device &dev, const State &s, const std::vector<FusedBlock> &blocks) {
+    std::vector<T> all(blocks.size() * 4096);
+    std::vector<T> one;
     for (size_t b = 0; b < blocks.size(); b++) {
         block5_lane_pack(blocks[b].M, one);
         std::copy(one.begin(), one.end(), all.begin() + static_cast<std::ptrdiff_t>(b * 4096));
     }
-    auto *d = static_cast<float *>(ensure_work(dev, all.size() * sizeof(float)));
-    PLK_HIP(hipMemcpyAsync(d, all.data(), all.size() * sizeof(float), hipMemcpyHostToDevice,
+    auto *d = static_cast<T *>(ensure_work(dev, all.size() * sizeof(T)));
+    PLK_HIP(hipMemcpyAsync(d, all.data(), all.size() * sizeof(T), hipMemcpyHostToDevice,
                            dev.stream));
     for (size_t b = 0; b < blocks.size(); b++) {
         launch_block5(dev, s, blocks[b].bits, d + b * 4096);
     }
     // The host buffer stays alive until its upload has been read.
     PLK_HIP(hipStreamSynchronize(dev.stream));
+}
+
+} // namespace
+
+void apply_blocks(Device &dev, const State &s, const std::vector<FusedBlock> &blocks) {
+    if (blocks.empty()) {
+        return;
+    }
+    if (s.dtype == PLK_C64) {
+        apply_blocks_t<float>(dev, s, blocks);
+    } else {
+        apply_blocks_t<double>(dev, s, blocks);
+    }
 }
 
 } // namespace plk

@@ -440,7 +440,7 @@ void parity_t(Device &dev, const State &s, uint64_t mask, cd even, cd odd) {
 
 void launch_dense_matrix(Device &dev, const State &s, const std::vector<size_t> &wires,
                          const std::vector<cd> &mat) {
-    if (s.dtype == PLK_C64 && wires.size() >= 4 && wires.size() <= 5 && s.n >= 9) {
+    if (wires.size() >= 4 && wires.size() <= 5 && s.n >= 9) {
         // 16x16 and 32x32 blocks run on the matrix cores.
         CompiledOp op;
         op.name = "matrix";

@@ -423,14 +423,6 @@ PauliTables tables_at(void *dst, const PauliBlob &b) {
     return t;
 }
 
-PauliTables upload_pauli(Device &dev, const std::vector<PauliTerm> &terms) {
-    const PauliBlob b = pack_pauli(terms);
-    void *blob = nullptr;
-    PLK_HIP(hipMalloc(&blob, b.bytes.size()));
-    PLK_HIP(hipMemcpy(blob, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
-    return tables_at(blob, b);
-}
-
 // Launches pauli_expval_kernel over grid blocks into partials.
 void launch_pauli_expval(Device &dev, const State &s, const PauliTables &t, unsigned grid,
                          double *partials) {
@@ -447,10 +439,6 @@ void launch_pauli_expval(Device &dev, const State &s, const PauliTables &t, unsi
     PLK_HIP(hipGetLastError());
 }
 
-void free_tables(Device &dev, void *blob) {
-    PLK_HIP(hipStreamSynchronize(dev.stream));
-    PLK_HIP(hipFree(blob));
-}
 
 } // namespace
 
@@ -463,18 +451,62 @@ void inner_products(Device &dev, const State *const *xs, int nx, const State &y,
     }
 }
 
+PauliDevice::~PauliDevice() {
+    if (blob != nullptr) {
+        (void)hipSetDevice(device);
+        (void)hipStreamSynchronize(stream);
+        (void)hipFree(blob);
+    }
+}
+
+std::unique_ptr<PauliDevice> pauli_upload(Device &dev, const std::vector<PauliTerm> &terms) {
+    auto t = std::make_unique<PauliDevice>();
+    t->device = dev.id;
+    t->stream = dev.stream;
+    if (terms.empty()) {
+        return t;
+    }
+    const PauliBlob b = pack_pauli(terms);
+    t->blob = dev_malloc(dev.id, b.bytes.size(), "Pauli tables");
+    PLK_HIP(hipMemcpy(t->blob, b.bytes.data(), b.bytes.size(), hipMemcpyHostToDevice));
+    t->toff = b.toff;
+    t->ngroups = b.ngroups;
+    return t;
+}
+
+namespace {
+
+PauliTables view_at(const void *blob, size_t toff, int ngroups) {
+    PauliTables v;
+    v.blob = const_cast<void *>(blob);
+    v.groups = static_cast<const PauliGroup *>(blob);
+    v.terms = reinterpret_cast<const PauliTermDev *>(static_cast<const unsigned char *>(blob) +
+                                                      toff);
+    v.ngroups = ngroups;
+    return v;
+}
+
+PauliTables view_of(const PauliDevice &t) { return view_at(t.blob, t.toff, t.ngroups); }
+
+} // namespace
+
+double pauli_expval(Device &dev, const State &s, const PauliDevice &t) {
+    if (t.ngroups == 0) {
+        return 0.0;
+    }
+    const unsigned grid = grid_for(dev, s.len);
+    ensure_red(dev, grid, 1);
+    launch_pauli_expval(dev, s, view_of(t), grid, dev.red);
+    double out = 0.0;
+    finish_sums(dev, static_cast<int>(grid), 1, &out);
+    return out;
+}
+
 double pauli_expval(Device &dev, const State &s, const std::vector<PauliTerm> &terms) {
     if (terms.empty()) {
         return 0.0;
     }
-    PauliTables t = upload_pauli(dev, terms);
-    const unsigned grid = grid_for(dev, s.len);
-    ensure_red(dev, grid, 1);
-    launch_pauli_expval(dev, s, t, grid, dev.red);
-    double out = 0.0;
-    finish_sums(dev, static_cast<int>(grid), 1, &out);
-    free_tables(dev, t.blob);
-    return out;
+    return pauli_expval(dev, s, *pauli_upload(dev, terms));
 }
 
 // ---------------------------------------------------------------------------
@@ -548,31 +580,48 @@ double pauli_expval_noalloc(Device &dev, const State &s, const std::vector<Pauli
     return out;
 }
 
+std::vector<unsigned char> pauli_image(const std::vector<PauliTerm> &terms, size_t &toff,
+                                       int &ngroups) {
+    PauliBlob b = pack_pauli(terms);
+    toff = b.toff;
+    ngroups = b.ngroups;
+    return std::move(b.bytes);
+}
+
+void launch_pauli_apply(Device &dev, const State &out, const State &in, const PauliDevice &td) {
+    launch_pauli_apply_at(dev, out, in, td.blob, td.toff, td.ngroups);
+}
+
+void launch_pauli_apply_at(Device &dev, const State &out, const State &in, const void *blob,
+                           size_t toff, int ngroups) {
+    if (ngroups == 0) {
+        launch_fill_zero(dev, out);
+        return;
+    }
+    const PauliTables t = view_at(blob, toff, ngroups);
+    const unsigned grid = grid_for(dev, in.len);
+    ProfScope prof("pauli_apply", static_cast<double>(t.ngroups + 1) * in.bytes(), dev.stream);
+    if (in.dtype == PLK_C64) {
+        hipLaunchKernelGGL((pauli_apply_kernel<float>), dim3(grid), dim3(kBlock), 0, dev.stream,
+                           static_cast<Cx<float> *>(out.d), static_cast<const Cx<float> *>(in.d),
+                           static_cast<uint64_t>(in.len), t.groups, t.ngroups, t.terms);
+    } else {
+        hipLaunchKernelGGL((pauli_apply_kernel<double>), dim3(grid), dim3(kBlock), 0,
+                           dev.stream, static_cast<Cx<double> *>(out.d),
+                           static_cast<const Cx<double> *>(in.d), static_cast<uint64_t>(in.len),
+                           t.groups, t.ngroups, t.terms);
+    }
+    PLK_HIP(hipGetLastError());
+}
+
 void launch_pauli_apply(Device &dev, const State &out, const State &in,
                         const std::vector<PauliTerm> &terms) {
     if (terms.empty()) {
         launch_fill_zero(dev, out);
         return;
     }
-    PauliTables t = upload_pauli(dev, terms);
-    const unsigned grid = grid_for(dev, in.len);
-    {
-        ProfScope prof("pauli_apply", static_cast<double>(t.ngroups + 1) * in.bytes(),
-                       dev.stream);
-        if (in.dtype == PLK_C64) {
-            hipLaunchKernelGGL((pauli_apply_kernel<float>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<Cx<float> *>(out.d),
-                               static_cast<const Cx<float> *>(in.d),
-                               static_cast<uint64_t>(in.len), t.groups, t.ngroups, t.terms);
-        } else {
-            hipLaunchKernelGGL((pauli_apply_kernel<double>), dim3(grid), dim3(kBlock), 0,
-                               dev.stream, static_cast<Cx<double> *>(out.d),
-                               static_cast<const Cx<double> *>(in.d),
-                               static_cast<uint64_t>(in.len), t.groups, t.ngroups, t.terms);
-        }
-        PLK_HIP(hipGetLastError());
-    }
-    free_tables(dev, t.blob);
+    // The temporary tables are freed (after a stream sync) when this returns.
+    launch_pauli_apply(dev, out, in, *pauli_upload(dev, terms));
 }
 
 CsrDevice::~CsrDevice() {

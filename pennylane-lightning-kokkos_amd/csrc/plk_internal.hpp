@@ -55,7 +55,21 @@ struct Device {
     double *jac = nullptr;        // grow-only Jacobian accumulator
     size_t jac_cap = 0;           // doubles
     void *noalloc_scratch = nullptr; // this device's copy of the fixed scratch array
+    // Per-call arena for uploaded plan tables: bump allocation, rewound at
+    // the first allocation of each entry-point call (stream order keeps a
+    // rewound region from being overwritten before earlier kernels read it).
+    void *arena = nullptr;
+    size_t arena_cap = 0;
+    size_t arena_used = 0;
+    uint64_t arena_call = 0;
+    std::vector<void *> arena_retired; // outgrown chunks, freed at the next rewind
 };
+
+// Marks the start of an entry-point call (rewinds every device's arena
+// lazily at its next allocation).
+void begin_call();
+// size bytes of the current call's arena on d (256-byte aligned).
+void *arena_alloc(Device &d, size_t bytes);
 
 Device &device(int id);
 // Serializes every entry point of the C ABI: the per-device scratch buffers
@@ -74,6 +88,9 @@ void set_device(int id);
 
 // Profiling: when enabled, launch() brackets kernels with HIP events.
 bool profiling_on();
+// Marks the device stream as being captured into a graph (profiling scopes
+// record nothing meanwhile).
+void set_capturing(bool on);
 void record_kernel(const char *family, float ms, double bytes);
 
 struct ProfScope {
@@ -416,6 +433,30 @@ void launch_pauli_apply(Device &dev, const State &out, const State &in,
                         const std::vector<PauliTerm> &terms);
 double pauli_expval(Device &dev, const State &s,
                     const std::vector<PauliTerm> &terms);
+// Device copy of a Pauli-word sum's grouped term tables. Observables keep one
+// per (device, qubit count) for their life, so repeated expectation values
+// and adjoint sweeps upload nothing.
+struct PauliDevice {
+    int device = -1;
+    hipStream_t stream = nullptr;
+    void *blob = nullptr;
+    size_t toff = 0; // byte offset of the term table
+    int ngroups = 0;
+    PauliDevice() = default;
+    PauliDevice(const PauliDevice &) = delete;
+    PauliDevice &operator=(const PauliDevice &) = delete;
+    ~PauliDevice();
+};
+std::unique_ptr<PauliDevice> pauli_upload(Device &dev, const std::vector<PauliTerm> &terms);
+double pauli_expval(Device &dev, const State &s, const PauliDevice &t);
+void launch_pauli_apply(Device &dev, const State &out, const State &in, const PauliDevice &t);
+// Host image of a Pauli-word sum's tables (the layout PauliDevice holds on
+// the device) and an application from tables already at `blob`.
+std::vector<unsigned char> pauli_image(const std::vector<PauliTerm> &terms, size_t &toff,
+                                       int &ngroups);
+void launch_pauli_apply_at(Device &dev, const State &out, const State &in, const void *blob,
+                           size_t toff, int ngroups);
+
 // Largest number of Pauli words pauli_expval_noalloc accepts.
 constexpr size_t kNoAllocMaxPauliWords = 4096;
 // pauli_expval with every device buffer (partials, total, term tables) in a
@@ -476,10 +517,12 @@ struct FusedBlock {
 // not; ops commute past skipped ops only when their wires are disjoint.
 // Blocks with fewer wires are widened with untouched low bits.
 std::vector<FusedBlock> fuse_blocks(const std::vector<CompiledOp> &ops, int n);
-// The 64 x 64 per-lane operand layout of block5_kernel for M.
+// The 64 x 64 per-lane operand layout of block5_kernel for M (FP32 for
+// complex64 states, FP64 for complex128 states).
 int block5_lane_pack(const std::vector<cd> &M, std::vector<float> &out);
-// One block pass of a complex64 state (n >= 9); d_apack = block5_lane_pack.
-void launch_block5(Device &dev, const State &s, const int *bits, const float *d_apack);
+int block5_lane_pack(const std::vector<cd> &M, std::vector<double> &out);
+// One block pass (n >= 9); d_apack = block5_lane_pack in the state's precision.
+void launch_block5(Device &dev, const State &s, const int *bits, const void *d_apack);
 // Uploads the lane packs of every block (one copy) and applies them in order.
 void apply_blocks(Device &dev, const State &s, const std::vector<FusedBlock> &blocks);
 

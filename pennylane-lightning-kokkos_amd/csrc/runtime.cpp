@@ -80,6 +80,41 @@ void ensure_red(Device &d, size_t partial_doubles, size_t out_doubles) {
     }
 }
 
+namespace {
+uint64_t g_call = 1;
+} // namespace
+
+void begin_call() { g_call++; }
+
+void *arena_alloc(Device &d, size_t bytes) {
+    if (d.arena_call != g_call) {
+        d.arena_call = g_call;
+        d.arena_used = 0;
+        if (!d.arena_retired.empty()) {
+            PLK_HIP(hipStreamSynchronize(d.stream));
+            for (void *p : d.arena_retired) {
+                PLK_HIP(hipFree(p));
+            }
+            d.arena_retired.clear();
+        }
+    }
+    const size_t need = (bytes + 255) & ~size_t{255};
+    if (d.arena_used + need > d.arena_cap) {
+        // Earlier regions of this call may still be read by queued kernels:
+        // the outgrown chunk is retired, not freed.
+        if (d.arena) {
+            d.arena_retired.push_back(d.arena);
+        }
+        const size_t cap = std::max(need + (size_t{1} << 20), 2 * d.arena_cap);
+        d.arena = dev_malloc(d.id, cap, "plan tables");
+        d.arena_cap = cap;
+        d.arena_used = 0;
+    }
+    void *p = static_cast<unsigned char *>(d.arena) + d.arena_used;
+    d.arena_used += need;
+    return p;
+}
+
 void *ensure_work(Device &d, size_t bytes) {
     if (bytes > d.work_cap) {
         PLK_HIP(hipStreamSynchronize(d.stream));
@@ -142,8 +177,15 @@ void resolve_pending() {
 
 bool profiling_on() { return g_prof_on; }
 
+namespace {
+bool g_capturing = false;
+} // namespace
+
+void set_capturing(bool on) { g_capturing = on; }
+
 ProfScope::ProfScope(const char *f, double b, hipStream_t s) : family(f), bytes(b), stream(s) {
-    if (!g_prof_on) {
+    // Event timing does not apply to work recorded into a graph.
+    if (!g_prof_on || g_capturing) {
         return;
     }
     PLK_HIP(hipEventCreate(&e0));
