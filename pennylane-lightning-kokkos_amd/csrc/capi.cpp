@@ -1558,6 +1558,10 @@ double expval_obs(Device &dev, const State &s, const ObsNode &o) {
         check_sparse_wires(o, s);
         return csr_expval(dev, s, o.csr_on(dev, s));
     }
+    if (o.kind == ObsNode::Hermitian) {
+        // <psi|M|psi> group by group, without a copy of the state.
+        return dense_expval(dev, s, o.wires, o.matrix);
+    }
     TmpState tmp(s.n, s.dtype, s.device);
     launch_copy(dev, tmp.s, s);
     apply_obs_general(dev, tmp.s, o);

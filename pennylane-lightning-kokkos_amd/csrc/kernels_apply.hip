@@ -440,6 +440,14 @@ void parity_t(Device &dev, const State &s, uint64_t mask, cd even, cd odd) {
 
 void launch_dense_matrix(Device &dev, const State &s, const std::vector<size_t> &wires,
                          const std::vector<cd> &mat) {
+    if (wires.size() > 8) {
+        // Up to 12 wires: out of place into a temporary, then back.
+        TmpState tmp(s.n, s.dtype, s.device);
+        dense_apply_into(dev, tmp.s, s, wires, mat);
+        launch_copy(dev, s, tmp.s);
+        PLK_HIP(hipStreamSynchronize(dev.stream));
+        return;
+    }
     if (wires.size() >= 4 && wires.size() <= 5 && s.n >= 9) {
         // 16x16 and 32x32 blocks run on the matrix cores.
         CompiledOp op;

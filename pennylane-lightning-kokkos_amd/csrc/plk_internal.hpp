@@ -503,6 +503,14 @@ double csr_expval_once(Device &dev, const State &s, const CsrView &m);
 // malformed matrices.
 void csr_spmv(Device &dev, const State &out, const State &in, const CsrDevice &m);
 
+// Dense matrices on up to kDenseMaxWires wires (kernels_dense.hip):
+// <psi|M|psi> in FP64, and out = M in (out distinct from in).
+constexpr size_t kDenseMaxWires = 12;
+double dense_expval(Device &dev, const State &s, const std::vector<size_t> &wires,
+                    const std::vector<cd> &mat);
+void dense_apply_into(Device &dev, const State &out, const State &in,
+                      const std::vector<size_t> &wires, const std::vector<cd> &mat);
+
 // Dense 5-qubit blocks on the matrix cores (kernels_block.hip, fuse.cpp).
 // A fused block: five ascending state bits and a 32x32 complex matrix over
 // the local index s = sum_t bit(bits[t]) << t (row-major).
