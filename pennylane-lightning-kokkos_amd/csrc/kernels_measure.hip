@@ -152,6 +152,10 @@ __global__ void __launch_bounds__(kBlock)
                         const PauliGroup *__restrict__ groups, int ngroups,
                         const PauliTermDev *__restrict__ terms,
                         double *__restrict__ partials) {
+    // H is Hermitian, so for an X mask x != 0 the terms at i and at i ^ x
+    // contribute equally: only indices whose highest bit of x is 0 are
+    // visited, counted twice (whole waves skip together when that bit is
+    // above the wave's index bits).
     double acc[1] = {0.0};
     for (uint64_t i = uint64_t{blockIdx.x} * blockDim.x + threadIdx.x; i < len;
          i += uint64_t{gridDim.x} * blockDim.x) {
@@ -159,6 +163,14 @@ __global__ void __launch_bounds__(kBlock)
         const double ar = a.r, ai = a.i;
         for (int gi = 0; gi < ngroups; gi++) {
             const PauliGroup g = groups[gi];
+            double w = 1.0;
+            if (g.x) {
+                const uint64_t top = uint64_t{1} << (63 - __clzll(static_cast<long long>(g.x)));
+                if (i & top) {
+                    continue;
+                }
+                w = 2.0;
+            }
             const Cx<T> b = g.x ? ld(psi, i ^ g.x) : a;
             double cr, ci;
             group_coeff(terms, g, i, cr, ci);
@@ -166,7 +178,7 @@ __global__ void __launch_bounds__(kBlock)
             const double br = b.r, bi = b.i;
             const double tr = cr * br - ci * bi;
             const double ti = cr * bi + ci * br;
-            acc[0] = fma(ar, tr, fma(ai, ti, acc[0]));
+            acc[0] = fma(w * ar, tr, fma(w * ai, ti, acc[0]));
         }
     }
     block_sum<1>(acc);
