@@ -309,7 +309,7 @@ def run_c1(capi, reps=50):
 MFMA_FP32_PEAK_TFLOPS = 157.3
 
 
-def run_c5(capi, n=32, layers=40):
+def run_c5(capi, n=32, layers=40, reference=True):
     """Large-state fused form: the 32-qubit complex64 circuit of 40 layers of
     Haar SU(2) on every wire and CZ bricks, fused into 5-qubit 32x32 blocks
     applied on the matrix cores; the same circuit on complex128 through the
@@ -340,6 +340,8 @@ def run_c5(capi, n=32, layers=40):
                                "hbm_frac": round(gbps / HBM_PEAK_GBS, 3), "mfma_TFLOPs": round(tfl, 1),
                                "mfma_frac": round(tfl / MFMA_FP32_PEAK_TFLOPS, 3)}
     log(f"bench: C5 blocks {1e3 * dt:.0f} ms, {nb} blocks, {k_ms:.2f} ms/block, {gbps:.0f} GB/s, {tfl:.1f} TFLOP/s")
+    if not reference:
+        return out
     e64 = a.expval(h)
     idx = np.random.default_rng(32).integers(0, 1 << n, size=4096)
     amps64 = _amplitudes(capi, a, idx)
@@ -496,6 +498,8 @@ def main():
     ap.add_argument("--c1", type=int, default=1, help="time the 12-qubit Jacobian latency (0 skips it)")
     ap.add_argument("--c2", type=int, default=1, help="run the gate-throughput workload (0 skips it)")
     ap.add_argument("--c5", type=int, default=1, help="run the 32-qubit fused-block workload (0 skips it)")
+    ap.add_argument("--c5-reference", type=int, default=1,
+                    help="also run the complex128 / complex64 plain paths of that workload (0: blocks only)")
     ap.add_argument("--batched", type=int, default=4096,
                     help="circuits of the batched workload (0 skips it)")
     ap.add_argument("--batched-seq", type=int, default=4096,
@@ -585,7 +589,7 @@ def main():
 
     if args.c5:
         log("bench: large-state fused form (32 qubits, MFMA blocks)")
-        extra["c5"] = run_c5(capi)
+        extra["c5"] = run_c5(capi, reference=bool(args.c5_reference))
 
     if args.batched > 0:
         log(f"bench: batched workload ({args.batched} circuits)")
