@@ -1403,7 +1403,6 @@ bool use_rtile(int n, int dtype) {
 
 void apply_single(Device &dev, const State &s, const CompiledOp &op);
 
-// Applies a forward plan (no observables) to s.
 // Launches a forward plan whose tables are at pl (no host synchronisation
 // unless a step is a single op outside the fused kernels).
 void enqueue_forward(Device &dev, const State &s, const std::vector<Step> &steps,
