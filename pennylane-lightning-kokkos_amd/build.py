@@ -33,7 +33,8 @@ HIP_FLAGS = [
 
 LIB_SOURCES = ["gates.cpp", "runtime.cpp", "capi.cpp", "fuse.cpp", "kernels_apply.hip",
                "kernels_measure.hip", "kernels_tile.hip", "kernels_batch.hip",
-               "kernels_sample.hip", "kernels_block.hip", "kernels_dense.hip"]
+               "kernels_sample.hip", "kernels_block.hip", "kernels_dense.hip",
+               "kernels_step.hip"]
 
 # The register-tile kernel is compiled once per (precision, observable chunk)
 # variant, each as an object of its own (RT_NJMAX in csrc/plk_internal.hpp).

@@ -870,12 +870,25 @@ struct RtProgram {
     size_t off = 0; // byte offset of the program in the plan blob
 };
 
+// A single-op step run by the step kernels (kernels_step.hip): its launch
+// arguments, the byte offset of its U and G matrices in the image's matrix
+// table, and its Jacobian column and coefficient.
+struct StepMeta {
+    bool ok = false;
+    StepLaunch L{};
+    size_t off = 0;
+    int col = 0;
+    double coef = 0.0;
+};
+
 struct PlanImage {
     std::vector<unsigned char> ops;
     std::vector<TileBundle> bundles;
     std::vector<SegMeta> metas;
     std::vector<unsigned char> rt;
     std::vector<RtProgram> progs;
+    std::vector<unsigned char> mats; // U and G of the step-kernel steps (FP64 complex)
+    std::vector<StepMeta> singles;
 };
 
 // Device addresses of an image's tables.
@@ -883,7 +896,142 @@ struct PlanPlace {
     const unsigned char *ops = nullptr;
     const TileBundle *bundles = nullptr;
     const unsigned char *rt = nullptr;
+    const unsigned char *mats = nullptr;
 };
+
+// The 2^m x 2^m matrix, over the wires uw (uw[0] most significant), of an
+// operator on the wires gw (a subset of uw) given as a dense matrix or as
+// parity phases (even / odd); false when gw is not a subset of uw.
+bool embed_matrix(const std::vector<size_t> &uw, const std::vector<size_t> &gw, OpKind kind,
+                  const std::vector<cd> &gm, cd even, cd odd, std::vector<cd> &out) {
+    const int m = static_cast<int>(uw.size());
+    const int k = static_cast<int>(gw.size());
+    const int D = 1 << m;
+    std::vector<int> shift; // bit of the local index that carries gw[q]
+    int gmask = 0;
+    for (auto w : gw) {
+        const auto it = std::find(uw.begin(), uw.end(), w);
+        if (it == uw.end()) {
+            return false;
+        }
+        shift.push_back(m - 1 - static_cast<int>(it - uw.begin()));
+        gmask |= 1 << shift.back();
+    }
+    auto sub = [&](int r) {
+        int v = 0;
+        for (int q = 0; q < k; q++) {
+            v |= ((r >> shift[q]) & 1) << (k - 1 - q);
+        }
+        return v;
+    };
+    out.assign(static_cast<size_t>(D) * D, cd{0.0, 0.0});
+    for (int r = 0; r < D; r++) {
+        for (int c = 0; c < D; c++) {
+            if ((r & ~gmask) != (c & ~gmask)) {
+                continue;
+            }
+            const int sr = sub(r), sc = sub(c);
+            cd v{0.0, 0.0};
+            if (kind == OpKind::Dense) {
+                v = gm[static_cast<size_t>(sr) * (size_t{1} << k) + sc];
+            } else if (sr == sc) {
+                v = (__builtin_popcount(sr) & 1) ? odd : even;
+            }
+            out[static_cast<size_t>(r) * D + c] = v;
+        }
+    }
+    return true;
+}
+
+// Fills m for a single-op step the step kernels can run: parity phases on
+// any number of wires (with a parity generator when trainable) or a dense
+// matrix on 3-4 wires whose generator acts inside its wires. Appends the
+// matrices to mats; returns false for anything else (the op then runs
+// through the per-state path).
+bool build_step(const PlanOp &p, int n, std::vector<unsigned char> &mats, StepMeta &m) {
+    if (env_int("PLK_STEPS", 1) == 0) {
+        return false;
+    }
+    const CompiledOp &U = p.U;
+    StepLaunch L{};
+    auto mask = [&](const std::vector<size_t> &w) {
+        uint64_t x = 0;
+        for (auto v : w) {
+            x |= uint64_t{1} << (n - 1 - static_cast<int>(v));
+        }
+        return x;
+    };
+    if (U.kind == OpKind::Parity) {
+        if (p.trainable && p.G.kind != OpKind::Parity) {
+            return false;
+        }
+        L.m = 0;
+        L.groups = uint64_t{1} << n;
+        L.umask = mask(U.wires);
+        L.gmask = p.trainable ? mask(p.G.wires) : 0;
+        L.ue[0] = U.even.real();
+        L.ue[1] = U.even.imag();
+        L.uo[0] = U.odd.real();
+        L.uo[1] = U.odd.imag();
+    } else if (U.kind == OpKind::Dense && U.wires.size() >= 3 && U.wires.size() <= STEP_MMAX) {
+        const int mw = static_cast<int>(U.wires.size());
+        const size_t D = size_t{1} << mw;
+        std::vector<cd> gm(D * D, cd{0.0, 0.0});
+        if (p.trainable &&
+            !embed_matrix(U.wires, p.G.wires, p.G.kind, p.G.mat, cd{1.0, 0.0}, cd{-1.0, 0.0}, gm)) {
+            return false;
+        }
+        L.m = mw;
+        L.groups = uint64_t{1} << (n - mw);
+        std::vector<int> bits;
+        for (auto w : U.wires) {
+            bits.push_back(n - 1 - static_cast<int>(w));
+        }
+        std::vector<int> sorted = bits;
+        std::sort(sorted.begin(), sorted.end());
+        for (int q = 0; q < mw; q++) {
+            L.bits[q] = sorted[q];
+        }
+        for (size_t k = 0; k < D; k++) {
+            uint64_t o = 0;
+            for (int q = 0; q < mw; q++) {
+                if ((k >> (mw - 1 - q)) & 1U) {
+                    o |= uint64_t{1} << bits[q];
+                }
+            }
+            L.off[k] = o;
+        }
+        mats.resize((mats.size() + 255) & ~size_t{255}, 0);
+        m.off = mats.size();
+        auto put = [&](const std::vector<cd> &M) {
+            for (const cd &v : M) {
+                const double re = v.real(), im = v.imag();
+                const auto *a = reinterpret_cast<const unsigned char *>(&re);
+                const auto *b = reinterpret_cast<const unsigned char *>(&im);
+                mats.insert(mats.end(), a, a + sizeof re);
+                mats.insert(mats.end(), b, b + sizeof im);
+            }
+        };
+        put(U.mat);
+        put(gm);
+    } else {
+        return false;
+    }
+    L.trainable = p.trainable ? 1 : 0;
+    L.write_phi = 1;
+    m.L = L;
+    m.col = p.col;
+    m.coef = p.coef;
+    m.ok = true;
+    return true;
+}
+
+// Whether the step kernels run this single-op step.
+bool step_kernel_ok(const PlanOp &p, int n) {
+    std::vector<unsigned char> scratch;
+    StepMeta m;
+    return build_step(p, n, scratch, m);
+}
 
 template <typename T>
 void build_seg_tables(const std::vector<Step> &steps, int n, PlanImage &img) {
@@ -1300,6 +1448,12 @@ PlanImage build_image(const std::vector<Step> &steps, const State &s) {
     } else {
         build_seg_tables<double>(steps, s.n, img);
     }
+    img.singles.resize(steps.size());
+    for (size_t i = 0; i < steps.size(); i++) {
+        if (!steps[i].fused && !steps[i].tiled) {
+            build_step(steps[i].single, s.n, img.mats, img.singles[i]);
+        }
+    }
     img.progs.resize(steps.size());
     const int R = rt_register_bits(s.dtype);
     for (size_t i = 0; i < steps.size(); i++) {
@@ -1313,7 +1467,8 @@ PlanImage build_image(const std::vector<Step> &steps, const State &s) {
 
 size_t image_bytes(const PlanImage &img) {
     auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
-    return up(img.ops.size()) + up(img.bundles.size() * sizeof(TileBundle)) + up(img.rt.size());
+    return up(img.ops.size()) + up(img.bundles.size() * sizeof(TileBundle)) + up(img.rt.size()) +
+           up(img.mats.size());
 }
 
 // Copies the image's tables to base (image_bytes(img) bytes) on the device's
@@ -1334,6 +1489,7 @@ PlanPlace copy_image(Device &dev, const PlanImage &img, unsigned char *base) {
     pl.bundles = reinterpret_cast<const TileBundle *>(
         put(img.bundles.data(), img.bundles.size() * sizeof(TileBundle)));
     pl.rt = put(img.rt.data(), img.rt.size());
+    pl.mats = put(img.mats.data(), img.mats.size());
     return pl;
 }
 
@@ -1344,6 +1500,7 @@ PlanPlace place_addresses(const PlanImage &img, unsigned char *base) {
     pl.ops = base;
     pl.bundles = reinterpret_cast<const TileBundle *>(base + up(img.ops.size()));
     pl.rt = base + up(img.ops.size()) + up(img.bundles.size() * sizeof(TileBundle));
+    pl.mats = pl.rt + up(img.rt.size());
     return pl;
 }
 
@@ -1359,8 +1516,15 @@ void append_signature(const PlanImage &img, std::vector<unsigned char> &sig) {
         const auto *b = static_cast<const unsigned char *>(p);
         sig.insert(sig.end(), b, b + bytes);
     };
-    const uint64_t sizes[3] = {img.ops.size(), img.bundles.size(), img.rt.size()};
+    const uint64_t sizes[4] = {img.ops.size(), img.bundles.size(), img.rt.size(), img.mats.size()};
     put(sizes, sizeof sizes);
+    for (const StepMeta &m : img.singles) {
+        put(&m.ok, sizeof m.ok);
+        put(&m.L, sizeof m.L);
+        put(&m.off, sizeof m.off);
+        put(&m.col, sizeof m.col);
+        put(&m.coef, sizeof m.coef);
+    }
     for (const SegMeta &m : img.metas) {
         put(&m.L, sizeof m.L);
         put(&m.first, sizeof m.first);
@@ -1426,6 +1590,12 @@ void enqueue_forward(Device &dev, const State &s, const std::vector<Step> &steps
             TL.write_phi = 1;
             TL.nslots = 0;
             launch_tile(dev, s, nullptr, pl.bundles + img.metas[i].first_bundle, pl.ops, TL);
+        } else if (img.singles[i].ok) {
+            StepLaunch L = img.singles[i].L;
+            L.nj = 0;
+            L.trainable = 0;
+            L.write_phi = 1;
+            launch_step(dev, s, nullptr, pl.mats + img.singles[i].off, L);
         } else {
             apply_single(dev, s, steps[i].single.U);
         }
@@ -1727,6 +1897,22 @@ bool enqueue_reverse_step(Device &dev, const State &lambda, void *const *hptr, s
         }
         return true;
     }
+    if (!st.fused && !st.tiled && img.singles[i].ok) {
+        const StepMeta &m = img.singles[i];
+        size_t j0 = 0;
+        for (int nj : chunk_sizes(n_obs, SEG_NJMAX)) {
+            StepLaunch L = m.L;
+            L.nj = nj;
+            L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
+            const int grid = launch_step(dev, lambda, hptr + j0, pl.mats + m.off, L);
+            if (L.trainable) {
+                launch_tile_reduce(dev, grid, 1, nj, static_cast<int>(j0), &m.col, &m.coef, d_jac,
+                                   static_cast<int>(n_tp));
+            }
+            j0 += nj;
+        }
+        return true;
+    }
     if (st.tiled) {
         const SegMeta &m = img.metas[i];
         size_t j0 = 0;
@@ -1853,9 +2039,11 @@ struct CaptureGuard {
     }
 };
 
-bool steps_have_single(const std::vector<Step> &steps) {
+// Whether a plan holds a single-op step outside the step kernels (those
+// synchronise with the host, so the plan cannot be captured).
+bool steps_have_single(const std::vector<Step> &steps, int n) {
     for (const auto &st : steps) {
-        if (!st.fused && !st.tiled) {
+        if (!st.fused && !st.tiled && !step_kernel_ok(st.single, n)) {
             return true;
         }
     }
@@ -1904,7 +2092,7 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
     };
     std::vector<PauliImg> pimg;
     bool graph = n <= kGraphMaxQubits && env_int("PLK_GRAPH", 1) != 0 &&
-                 !steps_have_single(steps) && !steps_have_single(fwd);
+                 !steps_have_single(steps, n) && !steps_have_single(fwd, n);
     for (size_t j = 0; graph && j < n_obs; j++) {
         std::vector<PauliTerm> terms;
         graph = to_pauli(*obs[j], n, terms);

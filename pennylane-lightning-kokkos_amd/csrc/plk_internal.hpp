@@ -317,6 +317,27 @@ struct RtLaunch {
 // the grid size.
 int launch_rtile(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
                  const RtLaunch &L);
+// Single-operation steps (kernels_step.hip) for operations the fused passes
+// do not hold. One launch applies U to phi (when write_phi) and to nj
+// observable states and, for a trainable op, writes the FP64 block sums of
+// Im<h_j | G phi> (phi before U) to dev.red as [j][block]; d_mats holds U then
+// G as 2^m x 2^m FP64 complex matrices (row-major, wires[0] most
+// significant). Returns the grid size.
+constexpr int STEP_MMAX = 4;
+struct StepLaunch {
+    int m;         // 3 or 4: dense U (and G) over the op's wires; 0: parity phases
+    int nj;        // observable states, 0..SEG_NJMAX
+    int write_phi;
+    int trainable;
+    int bits[STEP_MMAX];           // dense: ascending global bits of the wires
+    uint64_t groups;               // dense: 2^(n - m); parity: 2^n
+    uint64_t off[1 << STEP_MMAX];  // dense: global offset of local index k
+    uint64_t umask, gmask;         // parity: global bit masks of U's and G's wires
+    double ue[2], uo[2];           // parity: U's phase for even / odd parity
+};
+int launch_step(Device &dev, const State &phi, void *const *h_ptrs, const void *d_mats,
+                const StepLaunch &L);
+
 constexpr int RT_NJMAX = 5; // observable states per register-tile launch
 // One variant per precision (float / double) and chunk 0..RT_NJMAX, each
 // compiled from kernels_rtile.hip as an object of its own.
