@@ -945,9 +945,9 @@ bool embed_matrix(const std::vector<size_t> &uw, const std::vector<size_t> &gw, 
 
 // Fills m for a single-op step the step kernels can run: parity phases on
 // any number of wires (with a parity generator when trainable) or a dense
-// matrix on 3-4 wires whose generator acts inside its wires. Appends the
-// matrices to mats; returns false for anything else (the op then runs
-// through the per-state path).
+// matrix on at most 4 wires whose generator acts inside its wires, restricted
+// to the local states it moves. Appends the matrices to mats; returns false
+// for anything else (the op then runs through the per-state path).
 bool build_step(const PlanOp &p, int n, std::vector<unsigned char> &mats, StepMeta &m) {
     if (env_int("PLK_STEPS", 1) == 0) {
         return false;
@@ -973,15 +973,37 @@ bool build_step(const PlanOp &p, int n, std::vector<unsigned char> &mats, StepMe
         L.ue[1] = U.even.imag();
         L.uo[0] = U.odd.real();
         L.uo[1] = U.odd.imag();
-    } else if (U.kind == OpKind::Dense && U.wires.size() >= 3 && U.wires.size() <= STEP_MMAX) {
+    } else if (U.kind == OpKind::Dense && !U.wires.empty() && U.wires.size() <= STEP_MMAX) {
         const int mw = static_cast<int>(U.wires.size());
-        const size_t D = size_t{1} << mw;
-        std::vector<cd> gm(D * D, cd{0.0, 0.0});
+        const size_t full = size_t{1} << mw;
+        std::vector<cd> gm(full * full, cd{0.0, 0.0});
         if (p.trainable &&
             !embed_matrix(U.wires, p.G.wires, p.G.kind, p.G.mat, cd{1.0, 0.0}, cd{-1.0, 0.0}, gm)) {
             return false;
         }
+        // The local states the step has to touch: those where U differs
+        // from the identity or G is non-zero (exact comparisons: any
+        // deviation keeps a state in), padded to 2, 4, 8 or 16 with states
+        // on which U is the identity and G zero.
+        std::vector<size_t> act, idle;
+        for (size_t k = 0; k < full; k++) {
+            bool on = false;
+            for (size_t c = 0; c < full && !on; c++) {
+                const cd e = k == c ? cd{1.0, 0.0} : cd{0.0, 0.0};
+                on = U.mat[k * full + c] != e || U.mat[c * full + k] != e ||
+                     gm[k * full + c] != cd{0.0, 0.0} || gm[c * full + k] != cd{0.0, 0.0};
+            }
+            (on ? act : idle).push_back(k);
+        }
+        size_t D = 2;
+        while (D < act.size()) {
+            D *= 2;
+        }
+        for (size_t q = 0; act.size() < D; q++) {
+            act.push_back(idle[q]);
+        }
         L.m = mw;
+        L.dim = static_cast<int>(D);
         L.groups = uint64_t{1} << (n - mw);
         std::vector<int> bits;
         for (auto w : U.wires) {
@@ -992,24 +1014,27 @@ bool build_step(const PlanOp &p, int n, std::vector<unsigned char> &mats, StepMe
         for (int q = 0; q < mw; q++) {
             L.bits[q] = sorted[q];
         }
-        for (size_t k = 0; k < D; k++) {
+        for (size_t i = 0; i < D; i++) {
             uint64_t o = 0;
             for (int q = 0; q < mw; q++) {
-                if ((k >> (mw - 1 - q)) & 1U) {
+                if ((act[i] >> (mw - 1 - q)) & 1U) {
                     o |= uint64_t{1} << bits[q];
                 }
             }
-            L.off[k] = o;
+            L.off[i] = o;
         }
         mats.resize((mats.size() + 255) & ~size_t{255}, 0);
         m.off = mats.size();
         auto put = [&](const std::vector<cd> &M) {
-            for (const cd &v : M) {
-                const double re = v.real(), im = v.imag();
-                const auto *a = reinterpret_cast<const unsigned char *>(&re);
-                const auto *b = reinterpret_cast<const unsigned char *>(&im);
-                mats.insert(mats.end(), a, a + sizeof re);
-                mats.insert(mats.end(), b, b + sizeof im);
+            for (size_t r = 0; r < D; r++) {
+                for (size_t c = 0; c < D; c++) {
+                    const cd v = M[act[r] * full + act[c]];
+                    const double re = v.real(), im = v.imag();
+                    const auto *a = reinterpret_cast<const unsigned char *>(&re);
+                    const auto *b = reinterpret_cast<const unsigned char *>(&im);
+                    mats.insert(mats.end(), a, a + sizeof re);
+                    mats.insert(mats.end(), b, b + sizeof im);
+                }
             }
         };
         put(U.mat);

@@ -5,16 +5,21 @@
 //
 // The reference takes one such step as a copy mu = lambda, U^dagger on
 // lambda, G on mu, one blocking inner product per observable and U^dagger on
-// every H_j lambda (AdjointDiffKokkos.hpp:454-476): (7 + 3 n_obs) S of
-// traffic in 3 + 2 n_obs launches. Here one launch per chunk of observable
-// states does the whole step: each thread owns one group of 2^m amplitudes
-// (the op's wires, all other bits fixed), loads phi, forms G phi in
-// registers (phi before U, as the reference's mu), and for each observable
-// state h_j loads the group, adds Im<h_j|G phi> in FP64 and writes U h_j
-// back; phi itself is written once, with the last chunk:
-// (1 + w + 2 nj) S per launch. U and G are FP64 matrices read through the
-// scalar cache; amplitudes of complex64 states are widened to FP64 for the
-// products and rounded once on the store.
+// every H_j lambda (AdjointDiffKokkos.hpp:454-476): 3 + 2 n_obs launches.
+// Here one launch per chunk of observable states does the whole step. An op
+// on m wires acts on groups of 2^m amplitudes (the op's wires, all other
+// bits fixed), but most such ops move only a few of a group's states
+// (DoubleExcitation: 2 of 16, Toffoli and CSWAP: 2 of 8). The host passes
+// the D-state subspace outside which U is the identity and G is zero
+// (D = 2, 4, 8 or 16) with U and G restricted to it. Each thread owns one
+// group: it loads phi's D amplitudes, forms G phi in registers (phi before
+// U, as the reference's mu), stores U phi (with the last chunk), and for each
+// observable state h_j loads the group's D amplitudes, adds Im<h_j|G phi> in
+// FP64 and writes U h_j back: (1 + w + 2 nj) (D / 2^m) S per launch, in
+// whole cache lines when the op's wires include the lowest index bits. U and
+// G are FP64 matrices read through the scalar cache; amplitudes of complex64
+// states are widened to FP64 for the products and rounded once on the
+// store.
 //
 // Jacobian sums: per thread in LDS (one slot per observable state), then a
 // fixed-order block sum into dev.red as [j][block]; tile_reduce_kernel adds
@@ -69,11 +74,10 @@ __device__ __forceinline__ void mat_store(Cx<T> *p, uint64_t base, const uint64_
     }
 }
 
-template <typename T, int M>
+template <typename T, int D>
 __global__ void __launch_bounds__(kStepBlock)
     step_dense_kernel(Cx<T> *__restrict__ phi, SegPtrs<T> hp, const double2 *__restrict__ mats,
                       StepLaunch a, double *__restrict__ partials) {
-    constexpr int D = 1 << M;
     const double2 *U = mats;
     const double2 *G = mats + D * D;
     __shared__ double sacc[SEG_NJMAX * kStepBlock];
@@ -89,8 +93,10 @@ __global__ void __launch_bounds__(kStepBlock)
          g += uint64_t{gridDim.x} * kStepBlock) {
         uint64_t base = g;
 #pragma unroll
-        for (int q = 0; q < M; q++) {
-            base = insert_zero(base, a.bits[q]);
+        for (int q = 0; q < STEP_MMAX; q++) {
+            if (q < a.m) {
+                base = insert_zero(base, a.bits[q]);
+            }
         }
         double vr[D], vi[D];
 #pragma unroll
@@ -199,21 +205,29 @@ int launch_step_t(Device &dev, const State &phi, void *const *h_ptrs, const void
     ProfScope prof(L.nj > 0 ? "adjoint_step" : "apply_step", bytes, dev.stream);
     auto *p = static_cast<Cx<T> *>(phi.d);
     const auto *m = static_cast<const double2 *>(d_mats);
-    switch (L.m) {
+    switch (L.m == 0 ? 0 : L.dim) {
     case 0:
         hipLaunchKernelGGL((step_parity_kernel<T>), dim3(grid), dim3(kStepBlock), 0, dev.stream, p,
                            hp, L, dev.red);
         break;
-    case 3:
-        hipLaunchKernelGGL((step_dense_kernel<T, 3>), dim3(grid), dim3(kStepBlock), 0, dev.stream,
+    case 2:
+        hipLaunchKernelGGL((step_dense_kernel<T, 2>), dim3(grid), dim3(kStepBlock), 0, dev.stream,
                            p, hp, m, L, dev.red);
         break;
     case 4:
         hipLaunchKernelGGL((step_dense_kernel<T, 4>), dim3(grid), dim3(kStepBlock), 0, dev.stream,
                            p, hp, m, L, dev.red);
         break;
+    case 8:
+        hipLaunchKernelGGL((step_dense_kernel<T, 8>), dim3(grid), dim3(kStepBlock), 0, dev.stream,
+                           p, hp, m, L, dev.red);
+        break;
+    case 16:
+        hipLaunchKernelGGL((step_dense_kernel<T, 16>), dim3(grid), dim3(kStepBlock), 0, dev.stream,
+                           p, hp, m, L, dev.red);
+        break;
     default:
-        raise(PLK_ERR_INVALID, "single-op step kernels take 3 or 4 dense wires or parity phases");
+        raise(PLK_ERR_INVALID, "single-op step kernels take subspaces of 2, 4, 8 or 16 states");
     }
     PLK_HIP(hipGetLastError());
     return static_cast<int>(grid);
@@ -226,7 +240,8 @@ int launch_step(Device &dev, const State &phi, void *const *h_ptrs, const void *
     if (L.nj < 0 || L.nj > SEG_NJMAX) {
         raise(PLK_ERR_INVALID, "single-op step: observable chunk out of range");
     }
-    if (L.m != 0 && (L.m < 3 || L.m > STEP_MMAX || d_mats == nullptr)) {
+    if (L.m != 0 && (L.m < 1 || L.m > STEP_MMAX || d_mats == nullptr || L.dim < 2 ||
+                     L.dim > (1 << L.m))) {
         raise(PLK_ERR_INVALID, "single-op step: bad matrix width");
     }
     if (L.m > 0 && (uint64_t{1} << L.m) * L.groups != phi.len) {

@@ -321,17 +321,18 @@ int launch_rtile(Device &dev, const State &phi, void *const *h_ptrs, const void 
 // do not hold. One launch applies U to phi (when write_phi) and to nj
 // observable states and, for a trainable op, writes the FP64 block sums of
 // Im<h_j | G phi> (phi before U) to dev.red as [j][block]; d_mats holds U then
-// G as 2^m x 2^m FP64 complex matrices (row-major, wires[0] most
-// significant). Returns the grid size.
+// G restricted to the subspace, D x D FP64 complex matrices (row-major in
+// the order of off). Returns the grid size.
 constexpr int STEP_MMAX = 4;
 struct StepLaunch {
-    int m;         // 3 or 4: dense U (and G) over the op's wires; 0: parity phases
+    int m;         // 1..4: U (and G) on a subspace of the op's wires; 0: parity phases
     int nj;        // observable states, 0..SEG_NJMAX
     int write_phi;
     int trainable;
-    int bits[STEP_MMAX];           // dense: ascending global bits of the wires
-    uint64_t groups;               // dense: 2^(n - m); parity: 2^n
-    uint64_t off[1 << STEP_MMAX];  // dense: global offset of local index k
+    int dim;       // subspace states D (2, 4, 8, 16)
+    int bits[STEP_MMAX];           // ascending global bits of the wires
+    uint64_t groups;               // 2^(n - m); parity: 2^n
+    uint64_t off[1 << STEP_MMAX];  // global offset of subspace state k from the group base
     uint64_t umask, gmask;         // parity: global bit masks of U's and G's wires
     double ue[2], uo[2];           // parity: U's phase for even / odd parity
 };

@@ -2,7 +2,10 @@
 (DoubleExcitation±, Toffoli, CSWAP, MultiRZ on 3+ wires, 3-4-wire matrices)
 in the adjoint sweep and the forward application, against the oracle.
 
-Circuits mix those ops with fused one- and two-wire gates so every path
+Each step touches only the local states its op moves (2 of 16 for
+DoubleExcitation, 2 of 8 for Toffoli and CSWAP, padded subspaces for
+matrices that are the identity on part of their states). Circuits mix those
+ops with fused one- and two-wire gates so every path
 around them runs too: register segments (8 qubits), register tiles (12 and
 20 qubits complex128), LDS tiles (16 qubits), and HIP-graph replays. Nine
 observables exercise two observable chunks of a step launch.
@@ -132,3 +135,50 @@ def test_wide_only_circuit_trainable_subset(capi, monkeypatch):
     got = capi.adjoint_jacobian(s, [to_capi(capi, o) for o in obs], capi.Ops(*ops), tp, True)
     ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
     np.testing.assert_allclose(got, ref, rtol=1e-10, atol=1e-12)
+
+
+def _block_unitary(m, states, rng):
+    """Identity on 2^m states except a random unitary on `states`."""
+    U = np.eye(2**m, dtype=complex)
+    B = random_unitary(len(states), rng)
+    for a, r in enumerate(states):
+        for b, c in enumerate(states):
+            U[r, c] = B[a, b]
+    return U
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_matrices_on_state_subspaces(capi, monkeypatch, dtype):
+    """Matrices that move 2, 3, 4 or 5 of their local states (subspaces of
+    2, 4, 4 and 8 after padding), forward and in the adjoint sweep."""
+    monkeypatch.setenv("PLK_GRAPH", "0")
+    rng = np.random.default_rng(9500)
+    n = 13
+    names, params, wires, inverses, mats = [], [], [], [], []
+    blocks = [(3, [1, 6]), (4, [0, 5, 10]), (4, [3, 12, 7, 9]), (3, [0, 2, 4, 5, 7])]
+    for k in range(24):
+        m, states = blocks[k % len(blocks)]
+        names.append("QubitUnitary")
+        params.append([])
+        wires.append([int(x) for x in rng.choice(n, size=m, replace=False)])
+        inverses.append(bool(rng.integers(2)))
+        mats.append(_block_unitary(m, states, rng).reshape(-1))
+        g = ["RY", "RZ", "IsingXX"][k % 3]
+        names.append(g)
+        params.append([float(rng.uniform(0, 2 * np.pi))])
+        wires.append([int(x) for x in rng.choice(n, size=2 if g == "IsingXX" else 1, replace=False)])
+        inverses.append(False)
+        mats.append([])
+    ops = (names, params, wires, inverses, mats)
+    s = capi.State(n, dtype)
+    s.apply_ops(capi.Ops(*ops))
+    ref = O.apply_ops(O.zero_state(n), O.Ops(*ops))
+    np.testing.assert_allclose(s.download(), ref, atol=1e-12 if dtype == np.complex128 else 2e-5)
+    obs = _obs(n, rng, 3)
+    npar = sum(1 for p in params if p)
+    fresh = capi.State(n, dtype)
+    got = capi.adjoint_jacobian(fresh, [to_capi(capi, o) for o in obs], capi.Ops(*ops), list(range(npar)),
+                                True)
+    want = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops),
+                              list(range(npar)), True)
+    np.testing.assert_allclose(got, want, **_tol(dtype))
