@@ -290,6 +290,38 @@ def c5_workload(n=32, layers=40, seed=20261207):
     return n, (names, params, wires, [False] * len(names), mats), ("ham", coeffs, terms)
 
 
+def excitation_workload(n=20, doubles=120, singles=60, n_words=32, seed=20261209):
+    """A chemistry-style ansatz: the Hartree-Fock state (PauliX on the first
+    n/2 wires, the occupied orbitals), then `doubles` DoubleExcitation gates
+    on two occupied and two virtual wires and `singles` SingleExcitation gates
+    on one of each, interleaved (two doubles, then one single), angles
+    uniform in [0, 2 pi). Observables: one Hamiltonian of `n_words` random
+    Pauli words and PauliZ on wires 0..3."""
+    rng = np.random.default_rng(seed)
+    occ = list(range(n // 2))
+    virt = list(range(n // 2, n))
+    names, params, wires = [], [], []
+    for w in occ:
+        names.append("PauliX")
+        params.append([])
+        wires.append([w])
+    d = s_ = 0
+    while d < doubles or s_ < singles:
+        if (d + s_) % 3 == 2 and s_ < singles or d >= doubles:
+            names.append("SingleExcitation")
+            wires.append([int(rng.choice(occ)), int(rng.choice(virt))])
+            s_ += 1
+        else:
+            names.append("DoubleExcitation")
+            wires.append([int(x) for x in rng.choice(occ, 2, replace=False)] +
+                         [int(x) for x in rng.choice(virt, 2, replace=False)])
+            d += 1
+        params.append([float(rng.uniform(0, 2 * np.pi))])
+    ops = (names, params, wires, [False] * len(names), [[] for _ in names])
+    obs = [random_hamiltonian(n, n_words, rng)] + [("named", "PauliZ", [w]) for w in range(4)]
+    return n, ops, obs
+
+
 def c4_workload(count=4096, seed=20261205, nmin=6, nmax=16, dmin=2, dmax=8):
     """Independent circuits: n uniform in [nmin, nmax], depth uniform in
     [dmin, dmax] layers of RY on every wire, RZ on every wire, then the CNOT

@@ -182,3 +182,19 @@ def test_matrices_on_state_subspaces(capi, monkeypatch, dtype):
     want = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops),
                               list(range(npar)), True)
     np.testing.assert_allclose(got, want, **_tol(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_excitation_workload(capi, dtype):
+    """The benchmark's chemistry-style ansatz (20 qubits, 120 DoubleExcitation
+    + 60 SingleExcitation, 32-word Hamiltonian + 4 PauliZ) against the oracle
+    on the last 12 parameters and a few early ones."""
+    from helpers import excitation_workload
+
+    n, ops, obs = excitation_workload()
+    npar = sum(1 for p in ops[1] if p)
+    tp = [0, 1, 5] + list(range(npar - 12, npar))
+    s = capi.State(n, dtype)
+    got = capi.adjoint_jacobian(s, [to_capi(capi, o) for o in obs], capi.Ops(*ops), tp, True)
+    ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
+    np.testing.assert_allclose(got, ref, **_tol(dtype))
