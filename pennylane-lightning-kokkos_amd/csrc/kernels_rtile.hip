@@ -659,6 +659,10 @@ __global__ void __launch_bounds__(64)
                                                  std::make_integer_sequence<int, NS + 1>{});
         auto store = [&](auto RC) {
             using RT = Rot<NS, decltype(RC)::value>;
+            // The store layout's remaps (low index bits back onto lanes).
+            for (int f = 0; f < L.nfin; f++) {
+                remap_all<T, NS, A, R, decltype(RC)::value>(B, L.fin_rq[f], L.fin_rp[f]);
+            }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
                 if (s == 0 && !L.write_phi) {

@@ -310,6 +310,8 @@ struct RtLaunch {
     int tb[RT_TMAX];                   // tile bits, ascending global positions
     int reg_in[RT_RMAX], lane_in[RT_LANE_BITS];   // global bit of each slot at load
     int reg_out[RT_RMAX], lane_out[RT_LANE_BITS]; // and at store
+    int nfin;                          // remaps before the store (0-2):
+    int fin_rq[2], fin_rp[2];          // register bit fin_rq[f] <-> lane position fin_rp[f]
 };
 
 // Runs one register-tile pass over phi (and nj observable states, whose
