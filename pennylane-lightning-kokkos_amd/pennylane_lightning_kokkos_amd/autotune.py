@@ -1,14 +1,18 @@
-"""Per-device search of the tile and fusion widths of the adjoint engine.
+"""Per-device search of the pass kind, tile width and chunk width of the
+adjoint engine.
 
-`tune(n, dtype, n_obs)` times the engine on a synthetic layered circuit of
-the caller's size (RZ, RY, RZ on every wire, then a CNOT ring; n_obs PauliZ
-observables) for every candidate adjoint tile width, observable-chunk width
-and forward tile width, and returns the fastest setting as the environment
-variables the engine reads on every call (PLK_TILE_BITS_ADJ, PLK_TILE_NJ,
-PLK_TILE_BITS_FWD). With `apply=True` the winner is exported into this
-process's environment, so later calls use it; `cache` (a JSON file) keeps
-winners per (device, qubits, precision, observables). The settings change
-only how commuting operations are grouped and the order of FP64 sums.
+`tune(n, dtype, n_obs)` times full adjoint Jacobians on a synthetic layered
+circuit of the caller's size (RZ, RY, RZ on every wire, then a CNOT ring;
+n_obs PauliZ observables) for every candidate: register-tile passes with
+each observable-chunk width (PLK_RTILE=1, PLK_RTILE_NJ), and LDS tile passes
+with each adjoint tile width and chunk width (PLK_RTILE=0,
+PLK_TILE_BITS_ADJ, PLK_TILE_NJ). When the LDS passes win, the forward tile
+width (PLK_TILE_BITS_FWD) is searched as well. The fastest setting is
+returned as the environment variables the engine reads on every call; with
+`apply=True` it is exported into this process's environment, so later calls
+use it; `cache` (a JSON file) keeps winners per (device, qubits, precision,
+observables). The settings change only how commuting operations are grouped
+and the order of FP64 sums.
 """
 from __future__ import annotations
 
@@ -21,7 +25,11 @@ import numpy as np
 
 from . import _capi
 
-KNOBS = ("PLK_TILE_BITS_ADJ", "PLK_TILE_NJ", "PLK_TILE_BITS_FWD")
+KNOBS = ("PLK_RTILE", "PLK_RTILE_NJ", "PLK_TILE_BITS_ADJ", "PLK_TILE_NJ", "PLK_TILE_BITS_FWD")
+
+# Register bits of a register tile (complex128, complex64); a register-tile
+# pass needs at least that many qubits plus the six lane bits.
+_RT_REGISTER_BITS = {np.dtype(np.complex128): 3, np.dtype(np.complex64): 4}
 
 
 def _circuit(n: int, layers: int, seed: int):
@@ -55,12 +63,12 @@ def _key(n, dtype, n_obs) -> str:
 
 
 def tune(n: int, dtype=np.complex128, n_obs: int = 9, layers: int = 4, repeats: int = 2,
-         adj_bits: Sequence[int] = (9, 10, 11), chunk: Sequence[int] = (2, 3, 4),
-         fwd_bits: Sequence[int] = (9, 10, 11, 12), apply: bool = True,
-         cache: Optional[str] = None, device_id: int = 0) -> Dict[str, str]:
-    """Fastest (adjoint tile bits, chunk width, forward tile bits) for n
-    qubits; candidates outside what the engine accepts at this size are
-    skipped (tile bits at most n - 8, LDS budget)."""
+         rt_chunk: Sequence[int] = (2, 3, 4), adj_bits: Sequence[int] = (9, 10, 11),
+         chunk: Sequence[int] = (2, 3, 4), fwd_bits: Sequence[int] = (9, 10, 11, 12),
+         apply: bool = True, cache: Optional[str] = None, device_id: int = 0) -> Dict[str, str]:
+    """Fastest setting for n qubits; candidates outside what the engine
+    accepts at this size are skipped (register tiles need R + 6 qubits, LDS
+    tile bits at most n - 8, LDS budget)."""
     if cache and os.path.exists(cache):
         hit = json.load(open(cache)).get(_key(n, dtype, n_obs))
         if hit:
@@ -75,39 +83,44 @@ def tune(n: int, dtype=np.complex128, n_obs: int = 9, layers: int = 4, repeats: 
     saved = {k: os.environ.get(k) for k in KNOBS}
     best: Dict[str, str] = {}
     try:
-        timings = {}
-        for t in adj_bits:
-            if t > n - 8:
-                continue
-            for nj in chunk:
-                os.environ["PLK_TILE_BITS_ADJ"] = str(t)
-                os.environ["PLK_TILE_NJ"] = str(nj)
+        for k in KNOBS:
+            os.environ.pop(k, None)
+        cands = []
+        if n >= _RT_REGISTER_BITS[np.dtype(dtype)] + 6:
+            cands += [{"PLK_RTILE": "1", "PLK_RTILE_NJ": str(nj)} for nj in rt_chunk]
+        cands += [{"PLK_RTILE": "0", "PLK_TILE_BITS_ADJ": str(t), "PLK_TILE_NJ": str(nj)}
+                  for t in adj_bits if t <= n - 8 for nj in chunk]
+        timings = []
+        for c in cands:
+            os.environ.update(c)
+            try:
+                timings.append((_time(lambda: _capi.adjoint_jacobian(state, obs, ops, tp, True),
+                                      repeats), c))
+            except _capi.PLException:
+                pass
+            for k in c:
+                os.environ.pop(k, None)
+        if timings:
+            best = dict(min(timings, key=lambda tc: tc[0])[1])
+        if best.get("PLK_RTILE") == "0":
+            os.environ.update(best)
+            ftimes = {}
+            for t in fwd_bits:
+                if t > n - 8:
+                    continue
+                os.environ["PLK_TILE_BITS_FWD"] = str(t)
+
+                def fwd():
+                    state.reset()
+                    state.apply_ops(ops)
+                    state.synchronize()
+
                 try:
-                    timings[(t, nj)] = _time(
-                        lambda: _capi.adjoint_jacobian(state, obs, ops, tp, True), repeats)
+                    ftimes[t] = _time(fwd, repeats)
                 except _capi.PLException:
                     continue
-        if timings:
-            t, nj = min(timings, key=timings.get)
-            best["PLK_TILE_BITS_ADJ"], best["PLK_TILE_NJ"] = str(t), str(nj)
-            os.environ.update(best)
-        ftimes = {}
-        for t in fwd_bits:
-            if t > n - 8:
-                continue
-            os.environ["PLK_TILE_BITS_FWD"] = str(t)
-
-            def fwd():
-                state.reset()
-                state.apply_ops(ops)
-                state.synchronize()
-
-            try:
-                ftimes[t] = _time(fwd, repeats)
-            except _capi.PLException:
-                continue
-        if ftimes:
-            best["PLK_TILE_BITS_FWD"] = str(min(ftimes, key=ftimes.get))
+            if ftimes:
+                best["PLK_TILE_BITS_FWD"] = str(min(ftimes, key=ftimes.get))
     finally:
         for k, v in saved.items():
             if v is None:

@@ -13,13 +13,19 @@ def test_autotune_setting_cache_and_parity(capi, tmp_path, monkeypatch):
     from pennylane_lightning_kokkos_amd import autotune
 
     for k in autotune.KNOBS:
-        monkeypatch.setenv(k, "")
+        monkeypatch.delenv(k, raising=False)
     cache = str(tmp_path / "tune.json")
     best = autotune.tune(19, np.complex128, n_obs=3, layers=2, repeats=1, cache=cache)
-    assert best["PLK_TILE_BITS_ADJ"] in {"9", "10", "11"}
-    assert best["PLK_TILE_NJ"] in {"2", "3", "4"}
-    assert best["PLK_TILE_BITS_FWD"] in {"9", "10", "11"}
+    if best["PLK_RTILE"] == "1":
+        assert best["PLK_RTILE_NJ"] in {"2", "3", "4"}
+    else:
+        assert best["PLK_RTILE"] == "0"
+        assert best["PLK_TILE_BITS_ADJ"] in {"9", "10", "11"}
+        assert best["PLK_TILE_NJ"] in {"2", "3", "4"}
+        assert best["PLK_TILE_BITS_FWD"] in {"9", "10", "11"}
     assert autotune.tune(19, np.complex128, n_obs=3, cache=cache) == best  # cache hit
+    for k, v in best.items():
+        monkeypatch.setenv(k, v)
 
     names, params, wires = autotune._circuit(19, 2, seed=19)
     inv, mats = [False] * len(names), [[] for _ in names]

@@ -1,6 +1,7 @@
 """CSR expectation values and SpMV against the oracle.
 
-The kernels pick 4, 16 or 64 lanes per row from the average row length, take
+The kernels pick 4, 8, 16, 32 or 64 lanes per row from the average row
+length (each lane loads four entries per round), take
 int32 or int64 indices as handed over (int32 is the complex64 serialisation,
 _serialize.py:136-154) and check every row range and column index on the
 device. These tests cover each lane width, both index widths, both
@@ -40,7 +41,7 @@ def _tol(dtype):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("avg", [2, 12, 50])
+@pytest.mark.parametrize("avg", [2, 12, 50, 150, 300])
 @pytest.mark.parametrize("idx", [np.int32, np.int64])
 def test_csr_expval_lane_widths(capi, dtype, avg, idx):
     rng = np.random.default_rng(1000 + avg)
@@ -60,7 +61,7 @@ def test_csr_expval_lane_widths(capi, dtype, avg, idx):
 
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
-@pytest.mark.parametrize("avg", [3, 40])
+@pytest.mark.parametrize("avg", [3, 40, 150])
 @pytest.mark.parametrize("idx", [np.int32, np.int64])
 def test_csr_apply(capi, dtype, avg, idx):
     rng = np.random.default_rng(2000 + avg)
