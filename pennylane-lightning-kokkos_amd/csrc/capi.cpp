@@ -1639,7 +1639,7 @@ void enqueue_forward(Device &dev, const State &s, const std::vector<Step> &steps
             L.nj = 0;
             L.write_phi = 1;
             L.nslots = 0;
-            launch_rtile(dev, s, nullptr, pl.rt + img.progs[i].off, L);
+            launch_rtile(dev, s, nullptr, nullptr, pl.rt + img.progs[i].off, L);
         } else if (steps[i].fused) {
             SegLaunch L = img.metas[i].L;
             L.write_phi = 1;
@@ -1915,9 +1915,17 @@ struct Hasher {
 // Launches reverse step i (a fused segment, register-tile or LDS-tile pass
 // over lambda and every H_j lambda, with its Jacobian reductions); returns
 // false for a single op, which the caller runs.
-bool enqueue_reverse_step(Device &dev, const State &lambda, void *const *hptr, size_t n_obs,
+// The adjoint's lambda. Register-tile passes run their observable chunks
+// side by side, all reading cur, and write the updated lambda to alt; the
+// two then swap. Without alt (one chunk) they update cur in place.
+struct PhiPair {
+    State cur, alt;
+};
+
+bool enqueue_reverse_step(Device &dev, PhiPair &phi, void *const *hptr, size_t n_obs,
                           size_t n_tp, const Step &st, const PlanImage &img, size_t i,
                           const PlanPlace &pl, double *d_jac) {
+    const State &lambda = phi.cur;
     if (st.fused) {
         const SegMeta &m = img.metas[i];
         const int njmax = std::min(SEG_NJMAX, seg_max_nj(lambda.dtype, m.L.K));
@@ -1944,18 +1952,35 @@ bool enqueue_reverse_step(Device &dev, const State &lambda, void *const *hptr, s
         return true;
     }
     if (st.tiled && st.pass.rt) {
-        size_t j0 = 0;
-        for (int nj : chunk_sizes(n_obs, rtile_chunk(lambda.dtype))) {
+        // Consecutive chunks of equal width run side by side in one launch
+        // (up to RT_OBS_MAX states) when lambda has a second buffer.
+        const std::vector<int> sizes = chunk_sizes(n_obs, rtile_chunk(lambda.dtype));
+        size_t j0 = 0, ci = 0;
+        while (ci < sizes.size()) {
+            const int nj = sizes[ci];
+            int nc = 1;
+            while (phi.alt.d != nullptr && ci + nc < sizes.size() && sizes[ci + nc] == nj &&
+                   (nc + 1) * nj <= RT_OBS_MAX) {
+                nc++;
+            }
             RtLaunch L = img.progs[i].L;
             L.nj = nj;
-            L.write_phi = (j0 + nj == n_obs) ? 1 : 0;
-            const int grid = launch_rtile(dev, lambda, hptr + j0, pl.rt + img.progs[i].off, L);
+            L.nc = nc;
+            const size_t width = static_cast<size_t>(nc) * nj;
+            L.write_phi = (j0 + width == n_obs) ? 1 : 0;
+            const bool swap = L.write_phi && phi.alt.d != nullptr;
+            const int grid = launch_rtile(dev, phi.cur, swap ? &phi.alt : nullptr, hptr + j0,
+                                          pl.rt + img.progs[i].off, L);
             if (L.nslots > 0) {
-                launch_tile_reduce(dev, grid, L.nslots, nj, static_cast<int>(j0),
+                launch_tile_reduce(dev, grid, L.nslots, static_cast<int>(width), static_cast<int>(j0),
                                    st.pass.cols.data(), st.pass.coefs.data(), d_jac,
                                    static_cast<int>(n_tp));
             }
-            j0 += nj;
+            if (swap) {
+                std::swap(phi.cur, phi.alt);
+            }
+            j0 += width;
+            ci += nc;
         }
         return true;
     }
@@ -2007,7 +2032,7 @@ struct SweepGraph {
     hipStream_t stream = nullptr;
     const void *ref = nullptr;
     std::vector<unsigned char> sig;
-    std::unique_ptr<TmpState> lambda;
+    std::unique_ptr<TmpState> lambda, lambda2; // lambda2: second buffer of PhiPair (or none)
     std::vector<std::unique_ptr<TmpState>> H;
     std::vector<void *> hptr;
     void *tables = nullptr;
@@ -2046,7 +2071,21 @@ std::map<int, std::map<uint64_t, int>> &sweep_seen() {
 }
 
 size_t graph_bytes(const SweepGraph &g) {
-    return (1 + g.H.size()) * g.lambda->s.bytes();
+    return (1 + g.H.size() + (g.lambda2 ? 1 : 0)) * g.lambda->s.bytes();
+}
+
+// Whether the reverse plan runs register-tile chunks side by side, which
+// needs lambda in two buffers.
+bool needs_alt(const std::vector<Step> &steps, size_t n_obs, int dtype) {
+    if (n_obs <= static_cast<size_t>(rtile_chunk(dtype))) {
+        return false;
+    }
+    for (const auto &st : steps) {
+        if (st.tiled && st.pass.rt) {
+            return true;
+        }
+    }
+    return false;
 }
 
 // Everything but the angles: op names / wires / inverse flags / parameter
@@ -2205,7 +2244,7 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             for (const auto &c : list) {
                 held += graph_bytes(*c);
             }
-            const size_t need = (1 + n_obs) * ref.bytes();
+            const size_t need = (2 + n_obs) * ref.bytes();
             while (!list.empty() &&
                    (list.size() >= kGraphsPerDevice || held + need > kGraphBytesPerDevice)) {
                 held -= graph_bytes(*list.front());
@@ -2218,6 +2257,9 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             ng->ref = ref.d;
             ng->sig = sig;
             ng->lambda = std::make_unique<TmpState>(n, ref.dtype, ref.device);
+            if (needs_alt(steps, n_obs, ref.dtype)) {
+                ng->lambda2 = std::make_unique<TmpState>(n, ref.dtype, ref.device);
+            }
             for (size_t j = 0; j < n_obs; j++) {
                 ng->H.push_back(std::make_unique<TmpState>(n, ref.dtype, ref.device));
                 ng->hptr.push_back(ng->H.back()->s.d);
@@ -2229,7 +2271,9 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             ng->jac = static_cast<double *>(dev_malloc(dev.id, sizeof(double) * n_obs * n_tp, "graph Jacobian"));
             // The launchers grow the partials buffer on demand, which would
             // synchronise inside the capture: size it for any pass first.
-            ensure_red(dev, static_cast<size_t>(dev.num_cus) * 8 * std::max(TILE_PMAX, RT_PMAX) * 8,
+            ensure_red(dev,
+                       static_cast<size_t>(dev.num_cus) * 32 * std::max(TILE_PMAX, RT_PMAX) *
+                           std::max(RT_NJMAX, SEG_NJMAX),
                        256);
             auto *tb = static_cast<unsigned char *>(ng->tables);
             // Addresses only: the tables are copied before every replay.
@@ -2246,8 +2290,9 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
                     pt += (pimg[j].bytes.size() + 255) & ~size_t{255};
                 }
                 launch_fill_zero_doubles(dev, ng->jac, n_obs * n_tp);
+                PhiPair phi{lam, ng->lambda2 ? ng->lambda2->s : State{}};
                 for (size_t i = 0; i < steps.size(); i++) {
-                    enqueue_reverse_step(dev, lam, ng->hptr.data(), n_obs, n_tp, steps[i], img_r,
+                    enqueue_reverse_step(dev, phi, ng->hptr.data(), n_obs, n_tp, steps[i], img_r,
                                          i, pr, ng->jac);
                 }
                 hipGraph_t graph_def = cap.end();
@@ -2300,10 +2345,14 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
     std::vector<std::pair<size_t, double>> host_entries; // (flat index, value)
 
     const PlanPlace pl = place_in_arena(dev, img_r);
-    std::unique_ptr<TmpState> mu;
+    std::unique_ptr<TmpState> mu, lambda2;
+    if (needs_alt(steps, n_obs, ref.dtype)) {
+        lambda2 = std::make_unique<TmpState>(n, ref.dtype, ref.device);
+    }
+    PhiPair phi{lambda.s, lambda2 ? lambda2->s : State{}};
     for (size_t i = 0; i < steps.size(); i++) {
         const Step &st = steps[i];
-        if (enqueue_reverse_step(dev, lambda.s, hptr.data(), n_obs, n_tp, st, img_r, i, pl, d_jac)) {
+        if (enqueue_reverse_step(dev, phi, hptr.data(), n_obs, n_tp, st, img_r, i, pl, d_jac)) {
             continue;
         }
         const PlanOp &p = st.single;
@@ -2311,9 +2360,9 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             if (!mu) {
                 mu = std::make_unique<TmpState>(n, ref.dtype, ref.device);
             }
-            launch_copy(dev, mu->s, lambda.s);
+            launch_copy(dev, mu->s, phi.cur);
         }
-        apply_single(dev, lambda.s, p.U);
+        apply_single(dev, phi.cur, p.U);
         if (p.trainable) {
             apply_generator_dev(dev, mu->s, p.G);
             std::vector<double> ri(2 * n_obs);

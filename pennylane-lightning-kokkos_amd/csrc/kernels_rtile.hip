@@ -597,14 +597,29 @@ __device__ __forceinline__ void for_rot(int rot, F &&f, std::integer_sequence<in
     ((rot == RS ? (f(IC<RS>{}), 0) : 0), ...);
 }
 
+// Observable-state pointers of one launch: nc chunks of NJ states.
+template <typename T> struct RtPtrs {
+    Cx<T> *p[RT_OBS_MAX];
+};
+
 template <typename T, int NJ, typename Off>
 __global__ void __launch_bounds__(64)
-    rtile_kernel(Cx<T> *__restrict__ phi, SegPtrs<T> hp, const unsigned char *__restrict__ prog_g,
+    rtile_kernel(Cx<T> *phi, Cx<T> *phi_out, RtPtrs<T> hp, const unsigned char *__restrict__ prog_g,
                  RtLaunch L, double *__restrict__ partials) {
     constexpr int R = rt_r<T>();
     constexpr int A = 1 << R;
     constexpr int NS = NJ + 1;
     constexpr int NA = NJ > 0 ? NJ : 1;
+    // Block -> (chunk c, block b of that chunk). Ids are taken in groups of
+    // xgroup (consecutive ids land on different XCDs), so the nc blocks that
+    // stream the same tiles for different chunks share an XCD and its L2.
+    const int nc = L.nc;
+    const unsigned xg = static_cast<unsigned>(L.xgroup);
+    const unsigned kq = blockIdx.x / xg;
+    const int c = static_cast<int>(kq % static_cast<unsigned>(nc));
+    const unsigned b = (kq / static_cast<unsigned>(nc)) * xg + blockIdx.x % xg;
+    const unsigned bpc = gridDim.x / static_cast<unsigned>(nc);
+    const bool writes_phi = L.write_phi && c == nc - 1;
     __shared__ double lacc[NJ > 0 ? RT_PMAX * NA * 16 + 64 : 1]; // + one dummy slot per lane
     const int lane = static_cast<int>(__lane_id());
     if constexpr (NJ > 0) {
@@ -618,7 +633,7 @@ __global__ void __launch_bounds__(64)
     sp[0] = phi;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        sp[j + 1] = hp.p[j];
+        sp[j + 1] = hp.p[c * NJ + j];
     }
     // Per-lane offsets of the load and store layouts (amplitudes; the tile
     // base is added per tile); register slot k sits at the sum over its set
@@ -642,7 +657,7 @@ __global__ void __launch_bounds__(64)
         return o;
     };
     const uint64_t ntiles = uint64_t{1} << (L.n - L.T);
-    for (uint64_t g = blockIdx.x; g < ntiles; g += gridDim.x) {
+    for (uint64_t g = b; g < ntiles; g += bpc) {
         uint64_t base = g;
         for (int q = 0; q < L.T; q++) {
             base = insert_zero(base, L.tb[q]);
@@ -665,12 +680,13 @@ __global__ void __launch_bounds__(64)
             }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                if (s == 0 && !L.write_phi) {
+                if (s == 0 && !writes_phi) {
                     continue;
                 }
+                Cx<T> *dst = s == 0 ? phi_out : sp[s];
 #pragma unroll
                 for (int k = 0; k < A; k++) {
-                    sto(sp[s], static_cast<Off>((base + roff(L.reg_out, k)) * sizeof(Cx<T>)) + vout,
+                    sto(dst, static_cast<Off>((base + roff(L.reg_out, k)) * sizeof(Cx<T>)) + vout,
                         B[RT::in(s)][k]);
                 }
             }
@@ -686,7 +702,10 @@ __global__ void __launch_bounds__(64)
             for (int q = 0; q < 16; q++) {
                 s += lacc[i * 16 + q];
             }
-            partials[static_cast<uint64_t>(i) * gridDim.x + blockIdx.x] = s;
+            // Local (slot, j) -> launch-wide (slot, c * NJ + j).
+            const int slot = i / NJ, j = i % NJ;
+            const uint64_t row = static_cast<uint64_t>(slot) * (nc * NJ) + c * NJ + j;
+            partials[row * bpc + b] = s;
         }
     }
 }
@@ -697,10 +716,12 @@ __global__ void __launch_bounds__(64)
 // this file once per variant with PLK_RT_T and PLK_RT_NJ defined, so the
 // variants compile in parallel.
 template <typename T, int NJ>
-int rtile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
-                   const RtLaunch &L) {
-    SegPtrs<T> hp{};
-    for (int j = 0; j < NJ; j++) {
+int rtile_launch_t(Device &dev, const State &phi, void *phi_out, void *const *h_ptrs,
+                   const void *d_prog, const RtLaunch &Lin) {
+    RtLaunch L = Lin;
+    L.nc = std::max(1, L.nc);
+    RtPtrs<T> hp{};
+    for (int j = 0; j < NJ * L.nc; j++) {
         hp.p[j] = static_cast<Cx<T> *>(h_ptrs[j]);
     }
     const void *kfn = reinterpret_cast<const void *>(&rtile_kernel<T, NJ, uint64_t>);
@@ -712,21 +733,32 @@ int rtile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const voi
     }
     const uint64_t ntiles = uint64_t{1} << (L.n - L.T);
     const uint64_t cap = static_cast<uint64_t>(dev.num_cus) * blocks_per_cu;
-    const unsigned grid = static_cast<unsigned>(std::min(ntiles, cap));
+    // Blocks per chunk: the chunks share the resident blocks; in groups of 8
+    // (one per XCD) when there are enough.
+    uint64_t bpc = std::max<uint64_t>(1, std::min(ntiles, cap / static_cast<uint64_t>(L.nc)));
+    L.xgroup = 1;
+    if (bpc >= 8) {
+        bpc = bpc / 8 * 8;
+        L.xgroup = 8;
+    }
+    const unsigned grid = static_cast<unsigned>(bpc * static_cast<uint64_t>(L.nc));
     if (NJ > 0 && L.nslots > 0) {
         ensure_red(dev, static_cast<size_t>(grid) * L.nslots * NJ, 0);
     }
-    const double bytes = (1.0 + L.write_phi + 2.0 * NJ) * static_cast<double>(phi.bytes());
+    // Algorithmic bytes: phi read once and written once, every observable
+    // state read and written once.
+    const double bytes =
+        (1.0 + L.write_phi + 2.0 * NJ * L.nc) * static_cast<double>(phi.bytes());
     ProfScope prof(NJ > 0 ? "adjoint_rtile" : "apply_rtile", bytes, dev.stream);
     hipLaunchKernelGGL((rtile_kernel<T, NJ, uint64_t>), dim3(grid), dim3(64), 0, dev.stream,
-                       static_cast<Cx<T> *>(phi.d), hp, static_cast<const unsigned char *>(d_prog), L,
-                       dev.red);
+                       static_cast<Cx<T> *>(phi.d), static_cast<Cx<T> *>(phi_out), hp,
+                       static_cast<const unsigned char *>(d_prog), L, dev.red);
     PLK_HIP(hipGetLastError());
-    return static_cast<int>(grid);
+    return static_cast<int>(bpc);
 }
 
 #if defined(PLK_RT_T) && defined(PLK_RT_NJ)
-template int rtile_launch_t<PLK_RT_T, PLK_RT_NJ>(Device &, const State &, void *const *,
+template int rtile_launch_t<PLK_RT_T, PLK_RT_NJ>(Device &, const State &, void *, void *const *,
                                                  const void *, const RtLaunch &);
 #endif
 

@@ -304,9 +304,10 @@ struct RtLaunch {
     int n, T, R;
     int ninstr;
     int nslots;
-    int nj;
-    int write_phi;
-    int pad;
+    int nj;        // observable states per chunk
+    int write_phi; // the last chunk writes the updated phi (to phi_out)
+    int nc;        // observable chunks run side by side in this launch (>= 1)
+    int xgroup;    // block ids taken in groups of xgroup (8: one per XCD) when mapping to chunks
     int tb[RT_TMAX];                   // tile bits, ascending global positions
     int reg_in[RT_RMAX], lane_in[RT_LANE_BITS];   // global bit of each slot at load
     int reg_out[RT_RMAX], lane_out[RT_LANE_BITS]; // and at store
@@ -314,11 +315,15 @@ struct RtLaunch {
     int fin_rq[2], fin_rp[2];          // register bit fin_rq[f] <-> lane position fin_rp[f]
 };
 
-// Runs one register-tile pass over phi (and nj observable states, whose
-// FP64 partials per block go to dev.red as [slot * nj + j][block]); returns
-// the grid size.
-int launch_rtile(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
-                 const RtLaunch &L);
+// Runs one register-tile pass over phi and nc chunks of nj observable
+// states side by side (h_ptrs holds nc * nj state pointers). Every chunk
+// reads phi; the last one writes the updated phi to phi_out (nullptr: in
+// place, allowed only with nc == 1). FP64 partials per block go to dev.red
+// as [slot * (nc * nj) + j][block]; returns the blocks per chunk (the
+// reduction's grid).
+constexpr int RT_OBS_MAX = 24; // observable states per register-tile launch
+int launch_rtile(Device &dev, const State &phi, const State *phi_out, void *const *h_ptrs,
+                 const void *d_prog, const RtLaunch &L);
 // Single-operation steps (kernels_step.hip) for operations the fused passes
 // do not hold. One launch applies U to phi (when write_phi) and to nj
 // observable states and, for a trainable op, writes the FP64 block sums of
@@ -345,8 +350,8 @@ constexpr int RT_NJMAX = 5; // observable states per register-tile launch
 // One variant per precision (float / double) and chunk 0..RT_NJMAX, each
 // compiled from kernels_rtile.hip as an object of its own.
 template <typename T, int NJ>
-int rtile_launch_t(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
-                   const RtLaunch &L);
+int rtile_launch_t(Device &dev, const State &phi, void *phi_out, void *const *h_ptrs,
+                   const void *d_prog, const RtLaunch &L);
 
 // Batched circuits (kernels_batch.hip). One workgroup runs one circuit at a
 // time from |0...0>: forward passes, h = H phi and <H>, reverse passes with

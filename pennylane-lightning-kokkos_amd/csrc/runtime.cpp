@@ -359,28 +359,38 @@ namespace plk {
 
 namespace {
 template <typename T>
-int launch_rtile_t(Device &dev, const State &phi, void *const *h, const void *p, const RtLaunch &L) {
+int launch_rtile_t(Device &dev, const State &phi, void *out, void *const *h, const void *p,
+                   const RtLaunch &L) {
     switch (L.nj) {
-    case 0: return rtile_launch_t<T, 0>(dev, phi, h, p, L);
-    case 1: return rtile_launch_t<T, 1>(dev, phi, h, p, L);
-    case 2: return rtile_launch_t<T, 2>(dev, phi, h, p, L);
-    case 3: return rtile_launch_t<T, 3>(dev, phi, h, p, L);
-    case 4: return rtile_launch_t<T, 4>(dev, phi, h, p, L);
-    case 5: return rtile_launch_t<T, 5>(dev, phi, h, p, L);
+    case 0: return rtile_launch_t<T, 0>(dev, phi, out, h, p, L);
+    case 1: return rtile_launch_t<T, 1>(dev, phi, out, h, p, L);
+    case 2: return rtile_launch_t<T, 2>(dev, phi, out, h, p, L);
+    case 3: return rtile_launch_t<T, 3>(dev, phi, out, h, p, L);
+    case 4: return rtile_launch_t<T, 4>(dev, phi, out, h, p, L);
+    case 5: return rtile_launch_t<T, 5>(dev, phi, out, h, p, L);
     default: break;
     }
     raise(PLK_ERR_INVALID, "unsupported register-tile observable chunk " + std::to_string(L.nj));
 }
 } // namespace
 
-int launch_rtile(Device &dev, const State &phi, void *const *h_ptrs, const void *d_prog,
-                 const RtLaunch &L) {
+int launch_rtile(Device &dev, const State &phi, const State *phi_out, void *const *h_ptrs,
+                 const void *d_prog, const RtLaunch &L) {
     const int R = phi.dtype == PLK_C64 ? 4 : 3;
-    if (L.R != R || L.T != R + RT_LANE_BITS || L.T > L.n || L.nslots > RT_PMAX || L.nj < 0) {
+    const int nc = std::max(1, L.nc);
+    if (L.R != R || L.T != R + RT_LANE_BITS || L.T > L.n || L.nslots > RT_PMAX || L.nj < 0 ||
+        nc * std::max(1, L.nj) > RT_OBS_MAX || (nc > 1 && L.nj == 0)) {
         raise(PLK_ERR_INVALID, "register-tile pass does not match the state");
     }
-    return phi.dtype == PLK_C64 ? launch_rtile_t<float>(dev, phi, h_ptrs, d_prog, L)
-                                : launch_rtile_t<double>(dev, phi, h_ptrs, d_prog, L);
+    if (phi_out != nullptr && (phi_out->len != phi.len || phi_out->dtype != phi.dtype)) {
+        raise(PLK_ERR_INVALID, "register-tile pass: output state of another shape");
+    }
+    if (nc > 1 && L.write_phi && (phi_out == nullptr || phi_out->d == phi.d)) {
+        raise(PLK_ERR_INVALID, "register-tile pass: chunks side by side need a separate output");
+    }
+    void *out = phi_out ? phi_out->d : phi.d;
+    return phi.dtype == PLK_C64 ? launch_rtile_t<float>(dev, phi, out, h_ptrs, d_prog, L)
+                                : launch_rtile_t<double>(dev, phi, out, h_ptrs, d_prog, L);
 }
 
 } // namespace plk
