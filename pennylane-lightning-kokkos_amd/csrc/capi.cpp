@@ -1601,30 +1601,29 @@ void append_signature(const PlanImage &img, std::vector<unsigned char> &sig) {
     }
 }
 
-// Observable states per register-tile launch: 3 keeps phi plus three states
-// and the spare buffer at two wavefronts per SIMD (PLK_RTILE_NJ overrides, up
-// to RT_NJMAX).
-int rtile_chunk(int dtype) {
-    (void)dtype;
-    return std::max(1, std::min(RT_NJMAX, env_int("PLK_RTILE_NJ", 3)));
+// Observable states per register-tile chunk: 3 keeps phi plus three states
+// and the spare buffer at two wavefronts per SIMD. When a pass has fewer
+// tiles than the device has CUs, chunks of one state give the launch more
+// blocks (12 qubits complex128, 12 observables: 0.63 ms per Jacobian against
+// 0.74 with chunks of 3). PLK_RTILE_NJ overrides, up to RT_NJMAX.
+int rtile_chunk(int dtype, int n, int num_cus) {
+    const int e = env_int("PLK_RTILE_NJ", 0);
+    if (e > 0) {
+        return std::min(RT_NJMAX, e);
+    }
+    const int T = (dtype == PLK_C64 ? 4 : 3) + RT_LANE_BITS;
+    const bool few_tiles = n - T < 30 && (int64_t{1} << std::max(0, n - T)) < num_cus;
+    return few_tiles ? 1 : 3;
 }
 
-// Register-tile passes serve small states, where their few launches set the
-// latency, and states with at least as many tiles as CUs; between the two
-// the LDS tile passes measured faster (24-layer adjoint at 16 qubits
-// complex128: 2.12 vs 2.36 ms; at 10 and 12 qubits register tiles take 1.42
-// and 1.71 ms against 2.49 and 3.09). PLK_RTILE=0 turns them off,
-// PLK_RTILE=1 forces them from the tile width up.
+// Register-tile passes serve every state of at least one register tile:
+// with their observable chunks side by side they measured faster than the
+// LDS tile passes at every size from 10 to 21 qubits in both precisions
+// (6-layer adjoint, 9 observables, complex128: 0.70 vs 2.52 ms at 10 qubits,
+// 1.32 vs 2.12 at 16, 8.35 vs 11.26 at 21). PLK_RTILE=0 turns them off.
 bool use_rtile(int n, int dtype) {
     const int T = rt_register_bits(dtype) + RT_LANE_BITS;
-    const int mode = env_int("PLK_RTILE", -1);
-    if (mode == 0) {
-        return false;
-    }
-    if (mode == 1) {
-        return n >= T;
-    }
-    return n >= T && (n <= T + 5 || n >= T + 8);
+    return env_int("PLK_RTILE", -1) != 0 && n >= T;
 }
 
 void apply_single(Device &dev, const State &s, const CompiledOp &op);
@@ -1954,7 +1953,7 @@ bool enqueue_reverse_step(Device &dev, PhiPair &phi, void *const *hptr, size_t n
     if (st.tiled && st.pass.rt) {
         // Consecutive chunks of equal width run side by side in one launch
         // (up to RT_OBS_MAX states) when lambda has a second buffer.
-        const std::vector<int> sizes = chunk_sizes(n_obs, rtile_chunk(lambda.dtype));
+        const std::vector<int> sizes = chunk_sizes(n_obs, rtile_chunk(lambda.dtype, lambda.n, dev.num_cus));
         size_t j0 = 0, ci = 0;
         while (ci < sizes.size()) {
             const int nj = sizes[ci];
@@ -2076,8 +2075,8 @@ size_t graph_bytes(const SweepGraph &g) {
 
 // Whether the reverse plan runs register-tile chunks side by side, which
 // needs lambda in two buffers.
-bool needs_alt(const std::vector<Step> &steps, size_t n_obs, int dtype) {
-    if (n_obs <= static_cast<size_t>(rtile_chunk(dtype))) {
+bool needs_alt(const Device &dev, const std::vector<Step> &steps, size_t n_obs, int dtype, int n) {
+    if (n_obs <= static_cast<size_t>(rtile_chunk(dtype, n, dev.num_cus))) {
         return false;
     }
     for (const auto &st : steps) {
@@ -2257,7 +2256,7 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             ng->ref = ref.d;
             ng->sig = sig;
             ng->lambda = std::make_unique<TmpState>(n, ref.dtype, ref.device);
-            if (needs_alt(steps, n_obs, ref.dtype)) {
+            if (needs_alt(dev, steps, n_obs, ref.dtype, n)) {
                 ng->lambda2 = std::make_unique<TmpState>(n, ref.dtype, ref.device);
             }
             for (size_t j = 0; j < n_obs; j++) {
@@ -2346,7 +2345,7 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
 
     const PlanPlace pl = place_in_arena(dev, img_r);
     std::unique_ptr<TmpState> mu, lambda2;
-    if (needs_alt(steps, n_obs, ref.dtype)) {
+    if (needs_alt(dev, steps, n_obs, ref.dtype, n)) {
         lambda2 = std::make_unique<TmpState>(n, ref.dtype, ref.device);
     }
     PhiPair phi{lambda.s, lambda2 ? lambda2->s : State{}};

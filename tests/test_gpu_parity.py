@@ -246,8 +246,9 @@ def test_adjoint_errors(capi):
 
 
 # ----------------------------------------------------------------------------
-# LDS tile passes (states of 14+ qubits use them; PLK_TILES=0 forces the
-# register-segment path for cross-checks).
+# Fused passes at 14+ qubits: register-tile passes (the default) and LDS tile
+# passes (PLK_RTILE=0); PLK_RTILE=0 with PLK_TILES=0 forces register
+# segments for cross-checks.
 # ----------------------------------------------------------------------------
 def _layered_ops(n, layers, rng):
     names, params, wires = [], [], []
@@ -265,9 +266,15 @@ def _layered_ops(n, layers, rng):
     return names, params, wires, [False] * len(names), [[] for _ in names]
 
 
+@pytest.fixture(params=["rtile", "lds"])
+def engine(request, monkeypatch):
+    monkeypatch.setenv("PLK_RTILE", "1" if request.param == "rtile" else "0")
+    return request.param
+
+
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 @pytest.mark.parametrize("n", [14, 16, 18])
-def test_adjoint_tiles_layered(capi, n, dtype):
+def test_adjoint_tiles_layered(capi, engine, n, dtype):
     rng = np.random.default_rng(300 + n)
     ops = _layered_ops(n, 3, rng)
     obs = [("named", "PauliZ", [0]), ("named", "PauliX", [n - 1]), random_hamiltonian(n, 6, rng),
@@ -279,7 +286,7 @@ def test_adjoint_tiles_layered(capi, n, dtype):
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 @pytest.mark.parametrize("seed", range(3))
-def test_adjoint_tiles_random(capi, seed, dtype):
+def test_adjoint_tiles_random(capi, engine, seed, dtype):
     rng = np.random.default_rng(400 + seed)
     n = [15, 17, 18][seed]
     ops = random_ops(n, 120, rng, with_matrix=(seed != 1))
@@ -293,7 +300,7 @@ def test_adjoint_tiles_random(capi, seed, dtype):
 
 @pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
 @pytest.mark.parametrize("n", [20, 21])
-def test_apply_ops_tiles(capi, n, dtype):
+def test_apply_ops_tiles(capi, engine, n, dtype):
     rng = np.random.default_rng(500 + n)
     names, params, wires, inv, mats = random_ops(n, 150, rng, with_matrix=True)
     psi = random_state(n, rng)
@@ -312,7 +319,10 @@ def test_tiles_match_segments(capi, monkeypatch):
     s = _gpu_state(capi, O.zero_state(n), np.complex128)
     cobs = [to_capi(capi, o) for o in obs]
     cops = capi.Ops(*ops)
+    rtiled = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
+    monkeypatch.setenv("PLK_RTILE", "0")
     tiled = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
     monkeypatch.setenv("PLK_TILES", "0")
     segs = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
     np.testing.assert_allclose(tiled, segs, rtol=1e-11, atol=1e-13)
+    np.testing.assert_allclose(rtiled, segs, rtol=1e-11, atol=1e-13)
