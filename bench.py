@@ -621,8 +621,9 @@ def main():
     adj_ms = sum(v["total_ms"] for k, v in stats.items()
                  if k.startswith("adjoint_") or k.startswith("reduce")) / args.steps
     fwd_ms = sum(v["total_ms"] for k, v in stats.items() if k.startswith("apply_")) / args.steps
-    chunks = max(1, -(-len(obs) // 3))
-    passes = seg["launches"] / args.steps / chunks if seg["launches"] else None
+    # One register-tile launch is one pass for all observables (its chunks
+    # run side by side).
+    passes = seg["launches"] / args.steps if seg["launches"] else None
     extra = {"sweep": {"unfused_algorithmic_bytes": unfused, "adjoint_ms_per_step": round(adj_ms, 3),
                        "forward_ms_per_step": round(fwd_ms, 3),
                        "unfused_equivalent_GBps": round(unfused / (adj_ms * 1e-3) / 1e9, 1),
