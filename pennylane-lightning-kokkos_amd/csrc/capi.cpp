@@ -1951,21 +1951,25 @@ bool enqueue_reverse_step(Device &dev, PhiPair &phi, void *const *hptr, size_t n
         return true;
     }
     if (st.tiled && st.pass.rt) {
-        // Consecutive chunks of equal width run side by side in one launch
-        // (up to RT_OBS_MAX states) when lambda has a second buffer.
+        // The chunks (balanced: widths nj and nj - 1, wider first) run side
+        // by side in one launch, up to RT_OBS_MAX states, when lambda has a
+        // second buffer.
         const std::vector<int> sizes = chunk_sizes(n_obs, rtile_chunk(lambda.dtype, lambda.n, dev.num_cus));
         size_t j0 = 0, ci = 0;
         while (ci < sizes.size()) {
             const int nj = sizes[ci];
-            int nc = 1;
-            while (phi.alt.d != nullptr && ci + nc < sizes.size() && sizes[ci + nc] == nj &&
-                   (nc + 1) * nj <= RT_OBS_MAX) {
+            int nc = 1, nbig = 1;
+            size_t width = static_cast<size_t>(nj);
+            while (phi.alt.d != nullptr && ci + nc < sizes.size() &&
+                   width + static_cast<size_t>(sizes[ci + nc]) <= static_cast<size_t>(RT_OBS_MAX)) {
+                width += static_cast<size_t>(sizes[ci + nc]);
+                nbig += sizes[ci + nc] == nj ? 1 : 0;
                 nc++;
             }
             RtLaunch L = img.progs[i].L;
             L.nj = nj;
             L.nc = nc;
-            const size_t width = static_cast<size_t>(nc) * nj;
+            L.nbig = nbig;
             L.write_phi = (j0 + width == n_obs) ? 1 : 0;
             const bool swap = L.write_phi && phi.alt.d != nullptr;
             const int grid = launch_rtile(dev, phi.cur, swap ? &phi.alt : nullptr, hptr + j0,

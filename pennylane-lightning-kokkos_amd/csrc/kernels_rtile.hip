@@ -620,6 +620,11 @@ __global__ void __launch_bounds__(64)
     const unsigned b = (kq / static_cast<unsigned>(nc)) * xg + blockIdx.x % xg;
     const unsigned bpc = gridDim.x / static_cast<unsigned>(nc);
     const bool writes_phi = L.write_phi && c == nc - 1;
+    // This chunk's states and its first state in the launch: nbig chunks of
+    // NJ, then chunks of NJ - 1 (their last register state stays unused).
+    const int njc = c < L.nbig ? NJ : NJ - 1;
+    const int jc0 = c < L.nbig ? c * NJ : L.nbig * NJ + (c - L.nbig) * (NJ - 1);
+    const int width = L.nbig * NJ + (nc - L.nbig) * (NJ - 1);
     __shared__ double lacc[NJ > 0 ? RT_PMAX * NA * 16 + 64 : 1]; // + one dummy slot per lane
     const int lane = static_cast<int>(__lane_id());
     if constexpr (NJ > 0) {
@@ -633,7 +638,7 @@ __global__ void __launch_bounds__(64)
     sp[0] = phi;
 #pragma unroll
     for (int j = 0; j < NJ; j++) {
-        sp[j + 1] = hp.p[c * NJ + j];
+        sp[j + 1] = j < njc ? hp.p[jc0 + j] : phi;
     }
     // Per-lane offsets of the load and store layouts (amplitudes; the tile
     // base is added per tile); register slot k sits at the sum over its set
@@ -665,6 +670,13 @@ __global__ void __launch_bounds__(64)
         Cx<T> B[NS + 1][A];
 #pragma unroll
         for (int s = 0; s < NS; s++) {
+            if (s > njc) {
+#pragma unroll
+                for (int k = 0; k < A; k++) {
+                    B[s][k] = Cx<T>{T(0), T(0)};
+                }
+                continue;
+            }
 #pragma unroll
             for (int k = 0; k < A; k++) {
                 B[s][k] = ldo(sp[s], static_cast<Off>((base + roff(L.reg_in, k)) * sizeof(Cx<T>)) + vin);
@@ -680,7 +692,7 @@ __global__ void __launch_bounds__(64)
             }
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                if (s == 0 && !writes_phi) {
+                if ((s == 0 && !writes_phi) || s > njc) {
                     continue;
                 }
                 Cx<T> *dst = s == 0 ? phi_out : sp[s];
@@ -702,10 +714,12 @@ __global__ void __launch_bounds__(64)
             for (int q = 0; q < 16; q++) {
                 s += lacc[i * 16 + q];
             }
-            // Local (slot, j) -> launch-wide (slot, c * NJ + j).
+            // Local (slot, j) -> launch-wide (slot, jc0 + j).
             const int slot = i / NJ, j = i % NJ;
-            const uint64_t row = static_cast<uint64_t>(slot) * (nc * NJ) + c * NJ + j;
-            partials[row * bpc + b] = s;
+            if (j < njc) {
+                const uint64_t row = static_cast<uint64_t>(slot) * width + jc0 + j;
+                partials[row * bpc + b] = s;
+            }
         }
     }
 }
@@ -720,8 +734,13 @@ int rtile_launch_t(Device &dev, const State &phi, void *phi_out, void *const *h_
                    const void *d_prog, const RtLaunch &Lin) {
     RtLaunch L = Lin;
     L.nc = std::max(1, L.nc);
+    L.nbig = std::min(std::max(L.nbig, NJ > 0 ? 1 : 0), L.nc);
+    if (NJ == 0) {
+        L.nbig = L.nc;
+    }
+    const int width = L.nbig * NJ + (L.nc - L.nbig) * std::max(0, NJ - 1);
     RtPtrs<T> hp{};
-    for (int j = 0; j < NJ * L.nc; j++) {
+    for (int j = 0; j < width; j++) {
         hp.p[j] = static_cast<Cx<T> *>(h_ptrs[j]);
     }
     const void *kfn = reinterpret_cast<const void *>(&rtile_kernel<T, NJ, uint64_t>);
@@ -743,12 +762,11 @@ int rtile_launch_t(Device &dev, const State &phi, void *phi_out, void *const *h_
     }
     const unsigned grid = static_cast<unsigned>(bpc * static_cast<uint64_t>(L.nc));
     if (NJ > 0 && L.nslots > 0) {
-        ensure_red(dev, static_cast<size_t>(grid) * L.nslots * NJ, 0);
+        ensure_red(dev, static_cast<size_t>(bpc) * L.nslots * width, 0);
     }
     // Algorithmic bytes: phi read once and written once, every observable
     // state read and written once.
-    const double bytes =
-        (1.0 + L.write_phi + 2.0 * NJ * L.nc) * static_cast<double>(phi.bytes());
+    const double bytes = (1.0 + L.write_phi + 2.0 * width) * static_cast<double>(phi.bytes());
     ProfScope prof(NJ > 0 ? "adjoint_rtile" : "apply_rtile", bytes, dev.stream);
     hipLaunchKernelGGL((rtile_kernel<T, NJ, uint64_t>), dim3(grid), dim3(64), 0, dev.stream,
                        static_cast<Cx<T> *>(phi.d), static_cast<Cx<T> *>(phi_out), hp,

@@ -307,6 +307,7 @@ struct RtLaunch {
     int nj;        // observable states per chunk
     int write_phi; // the last chunk writes the updated phi (to phi_out)
     int nc;        // observable chunks run side by side in this launch (>= 1)
+    int nbig;      // the first nbig chunks hold nj states, the others nj - 1
     int xgroup;    // block ids taken in groups of xgroup (8: one per XCD) when mapping to chunks
     int tb[RT_TMAX];                   // tile bits, ascending global positions
     int reg_in[RT_RMAX], lane_in[RT_LANE_BITS];   // global bit of each slot at load
@@ -315,12 +316,13 @@ struct RtLaunch {
     int fin_rq[2], fin_rp[2];          // register bit fin_rq[f] <-> lane position fin_rp[f]
 };
 
-// Runs one register-tile pass over phi and nc chunks of nj observable
-// states side by side (h_ptrs holds nc * nj state pointers). Every chunk
+// Runs one register-tile pass over phi and nc chunks of observable states
+// side by side: nbig chunks of nj states, then nc - nbig chunks of nj - 1
+// (h_ptrs holds their pointers in chunk order). Every chunk
 // reads phi; the last one writes the updated phi to phi_out (nullptr: in
 // place, allowed only with nc == 1). FP64 partials per block go to dev.red
-// as [slot * (nc * nj) + j][block]; returns the blocks per chunk (the
-// reduction's grid).
+// as [slot * width + j][block], width = the launch's observable states;
+// returns the blocks per chunk (the reduction's grid).
 constexpr int RT_OBS_MAX = 24; // observable states per register-tile launch
 int launch_rtile(Device &dev, const State &phi, const State *phi_out, void *const *h_ptrs,
                  const void *d_prog, const RtLaunch &L);

@@ -379,7 +379,8 @@ int launch_rtile(Device &dev, const State &phi, const State *phi_out, void *cons
     const int R = phi.dtype == PLK_C64 ? 4 : 3;
     const int nc = std::max(1, L.nc);
     if (L.R != R || L.T != R + RT_LANE_BITS || L.T > L.n || L.nslots > RT_PMAX || L.nj < 0 ||
-        nc * std::max(1, L.nj) > RT_OBS_MAX || (nc > 1 && L.nj == 0)) {
+        nc * std::max(1, L.nj) > RT_OBS_MAX || (nc > 1 && L.nj == 0) || L.nbig > nc ||
+        (L.nj == 1 && L.nbig < nc)) {
         raise(PLK_ERR_INVALID, "register-tile pass does not match the state");
     }
     if (phi_out != nullptr && (phi_out->len != phi.len || phi_out->dtype != phi.dtype)) {

@@ -107,3 +107,26 @@ def test_rtile_matches_lds_tiles(capi, monkeypatch):
     s.upload(psi)
     b = capi.adjoint_jacobian(s, cobs, cops, list(range(npar)), True)
     np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-13)
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n,nj", [(12, "3"), (12, "1"), (18, "")])
+@pytest.mark.parametrize("n_obs", [2, 4, 5, 7, 8, 10, 25])
+def test_side_by_side_chunks(capi, monkeypatch, n, nj, n_obs, dtype):
+    """A pass runs its observable chunks side by side in one launch: chunks
+    of nj and nj - 1 states (5 observables in chunks of 3 -> 3 + 2), launches
+    split at 24 states (25 observables), lambda alternating between two
+    buffers; against the oracle."""
+    if nj:
+        monkeypatch.setenv("PLK_RTILE_NJ", nj)
+    monkeypatch.setenv("PLK_GRAPH", "0")
+    rng = np.random.default_rng(9700 + 31 * n + n_obs)
+    ops = random_ops(n, 50, rng, names=ONE_TWO_WIRE)
+    obs = [("named", ("PauliX", "PauliY", "PauliZ")[k % 3], [int(rng.integers(n))]) for k in range(n_obs - 1)]
+    obs.append(random_hamiltonian(n, 4, rng))
+    npar = sum(1 for p in ops[1] if p)
+    tp = list(range(npar)) if n <= 12 else list(range(npar - 12, npar))
+    s = capi.State(n, dtype)
+    got = capi.adjoint_jacobian(s, [to_capi(capi, o) for o in obs], capi.Ops(*ops), tp, True)
+    ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
+    _assert_jac(ref, got, dtype)
