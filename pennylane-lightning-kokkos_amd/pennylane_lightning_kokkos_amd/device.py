@@ -22,6 +22,8 @@ Kept from the reference:
   shifted by one when the tape starts with a state preparation;
 - `vjp` (`lightning_kokkos.py:689-728`): dy^T J as one adjoint sweep of the
   Hamiltonian sum_j dy_j O_j;
+- `batch_adjoint_jacobian`: the per-tape loop of a batch, with the tapes
+  the batched engine takes run in one launch;
 - `probability` rejects out-of-order wires as the device does
   (`tests/test_probs.py:116-191`); the engine itself supports them.
 """
@@ -342,6 +344,45 @@ class LightningKokkosAMD:
                                           False)
             jac[:, record_rows] = part
         return jac
+
+    def batch_adjoint_jacobian(self, tapes: Sequence[Tape]) -> List[np.ndarray]:
+        """Jacobians of many tapes, each run from |0...0> (the reference device
+        loops `adjoint_jacobian` over the tapes of a batch,
+        lightning_kokkos.py:621-687 per tape). Tapes the batched engine takes
+        -- no state preparation, no explicit matrices, operations on at most
+        two wires, one observable that is a sum of Pauli words -- run together
+        in one `plk_adjoint_batched` launch in the device's precision; the
+        others go through `adjoint_jacobian`. Returns one array per tape, as
+        `adjoint_jacobian` returns it."""
+        out: List[Optional[np.ndarray]] = [None] * len(tapes)
+        batch_ops, batch_obs, batch_tp, where = [], [], [], []
+        for k, tape in enumerate(tapes):
+            self._check_observables(tape.observables)
+            trainable = tape.get_trainable()
+            if not trainable:
+                out[k] = np.array(0)
+                continue
+            self._check_operations(tape.operations)
+            mapped = [Op(o.name, self._map(o.wires), list(o.params), o.inverse, o.matrix)
+                      for o in tape.operations]
+            ser, uses_sp = serialize_ops(Tape(mapped, []))
+            names, params, wires, inverses, mats = ser
+            eligible = (not uses_sp and len(tape.observables) == 1 and
+                        _pauli_only(tape.observables[0]) and
+                        all(len(w) <= 2 for w in wires) and all(len(m) == 0 for m in mats))
+            if not eligible:
+                out[k] = self.adjoint_jacobian(tape)
+                continue
+            batch_ops.append(_capi.Ops(names, params, wires, inverses, mats))
+            batch_obs.append(self._obs(tape.observables[0]))
+            batch_tp.append(trainable)
+            where.append(k)
+        if where:
+            _, rows = _capi.adjoint_batched([self.num_wires] * len(where), batch_ops, batch_obs,
+                                            batch_tp, dtype=self.C_DTYPE)
+            for k, row in zip(where, rows):
+                out[k] = np.asarray(row, dtype=np.float64).reshape(1, -1)
+        return out
 
     def vjp(self, observables, dy, starting_state=None, use_device_state=False):
         """Processing function tape -> dy^T J (one sweep of sum_j dy_j O_j)."""

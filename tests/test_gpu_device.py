@@ -126,3 +126,49 @@ def test_basis_state_and_subset_stateprep():
     assert st[0b101] == 1
     dev.apply([Op("StatePrep", [2], [np.array([0, 1])])])
     assert dev.state[0b001] == 1
+
+
+@pytest.mark.parametrize("c_dtype, tol", [(np.complex128, 1e-12), (np.complex64, 2e-5)])
+def test_batch_adjoint_jacobian_matches_per_tape(c_dtype, tol):
+    """batch_adjoint_jacobian: eligible tapes (Rot / RY / CNOT / IsingXX, one
+    Pauli-sum observable, trainable subsets) run in one batched launch, the
+    others (a Toffoli, two observables, a state preparation, an explicit
+    matrix, no trainable parameter) through adjoint_jacobian; every result
+    equals the per-tape call."""
+    rng = np.random.default_rng(9900)
+    n = 6
+    dev = LightningKokkosAMD(n, c_dtype=c_dtype)
+
+    def rand_ops(k):
+        ops = []
+        for _ in range(k):
+            g = ["Rot", "RY", "CNOT", "IsingXX", "RZ"][int(rng.integers(5))]
+            if g == "Rot":
+                ops.append(Op(g, [int(rng.integers(n))], [float(x) for x in rng.uniform(0, 6, 3)]))
+            elif g in ("RY", "RZ"):
+                ops.append(Op(g, [int(rng.integers(n))], [float(rng.uniform(0, 6))]))
+            else:
+                w = [int(x) for x in rng.choice(n, 2, replace=False)]
+                ops.append(Op(g, w, [float(rng.uniform(0, 6))] if g == "IsingXX" else []))
+        return ops
+
+    ham = ("ham", [0.5, -1.2, 0.3], [("named", "PauliZ", [0]),
+                                     ("tensor", [("named", "PauliX", [1]), ("named", "PauliY", [3])]),
+                                     ("named", "PauliX", [5])])
+    tapes = []
+    for k in range(6):
+        ops = rand_ops(10 + k)
+        npar = sum(len(o.params) for o in ops)
+        tp = None if k % 2 == 0 else sorted(int(x) for x in rng.choice(npar, npar // 2, replace=False))
+        tapes.append(Tape(ops, [ham if k % 3 else ("named", "PauliZ", [k % n])], tp))
+    tapes.append(Tape(rand_ops(8) + [Op("Toffoli", [0, 1, 2], [])], [ham]))
+    tapes.append(Tape(rand_ops(8), [ham, ("named", "PauliZ", [2])]))
+    tapes.append(Tape([Op("BasisState", [0, 1], [np.array([1, 0])])] + rand_ops(8), [ham]))
+    q, _ = np.linalg.qr(rng.normal(size=(4, 4)) + 1j * rng.normal(size=(4, 4)))
+    tapes.append(Tape(rand_ops(6) + [Op("QubitUnitary", [1, 4], [], False, q)], [ham]))
+    tapes.append(Tape([Op("CNOT", [0, 1], [])], [ham]))
+    got = dev.batch_adjoint_jacobian(tapes)
+    for tape, g in zip(tapes, got):
+        want = dev.adjoint_jacobian(tape)
+        assert np.shape(g) == np.shape(want)
+        np.testing.assert_allclose(g, want, atol=tol, rtol=0)
