@@ -454,7 +454,10 @@ std::vector<Step> plan_tiles(std::vector<PlanOp> &&ops_in, int n, int T, int L,
                 break;
             }
             const bool tr = all[i].trainable;
-            const uint64_t nb = bits | bmask[i];
+            // Register-tile passes need only the bits an op mixes in the
+            // tile: its other bits may lie outside, where they are fixed per
+            // tile and select the op's block.
+            const uint64_t nb = bits | (bundled ? bmask[i] : mmask[i]);
             const bool ok = taken.size() < kPassMaxOps && (!tr || nslots < pmax) &&
                             !(blocked & wmask[i]) && __builtin_popcountll(nb) <= T &&
                             __builtin_popcountll(mixed | mmask[i]) <= maxmix;
@@ -1306,6 +1309,19 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
         }
         raise(PLK_ERR_INVALID, "register-tile op outside the tile bits");
     };
+    // Selector of a non-mixing bit: its lane position, or 64 + the bit when
+    // it lies outside the tile (fixed for the whole tile).
+    auto sel_of = [&](int q) -> int {
+        for (int p = 0; p < RT_LANE_BITS; p++) {
+            if (lane[p] == q) {
+                return p;
+            }
+        }
+        if (std::find(ps.tb.begin(), ps.tb.end(), q) != ps.tb.end()) {
+            raise(PLK_ERR_INVALID, "register-tile selector bit sits in a register");
+        }
+        return 64 + q;
+    };
     auto next_use = [&](int q, size_t from) {
         for (size_t j = from; j < nops; j++) {
             if (std::find(mix[j].begin(), mix[j].end(), q) != mix[j].end()) {
@@ -1368,7 +1384,7 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
                 rt_encode_reg1<T>(b, in, p.U.mat, G);
             } else {
                 in.code = RT_LANE;
-                in.s0 = static_cast<int8_t>(lane_of(gbits[i][0]));
+                in.s0 = static_cast<int8_t>(sel_of(gbits[i][0]));
                 in.uoff = b.put(cplx({p.U.mat[0], p.U.mat[3]}));
                 if (G) {
                     in.goff = b.put({(*G)[0].real(), (*G)[3].real()});
@@ -1421,7 +1437,7 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
                 const int lr = 1 - k_reg; // local bit of the register wire
                 in.code = RT_SEL;
                 in.a = static_cast<int8_t>(k_reg == 0 ? r0 : r1);
-                in.s0 = static_cast<int8_t>(lane_of(gbits[i][1 - k_reg]));
+                in.s0 = static_cast<int8_t>(sel_of(gbits[i][1 - k_reg]));
                 const std::vector<cd> B0 = block2(p.U.mat, lr, 0), B1 = block2(p.U.mat, lr, 1);
                 if (is_diag2(B0) && is_diag2(B1)) {
                     in.ukind = U_DIAG;
@@ -1455,8 +1471,8 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
                 // Both wires on lanes: diagonal; block c = bit(wires[1]) + 2 bit(wires[0]) is
                 // the op's local index.
                 in.code = RT_LANE;
-                in.s0 = static_cast<int8_t>(lane_of(gbits[i][1]));
-                in.s1 = static_cast<int8_t>(lane_of(gbits[i][0]));
+                in.s0 = static_cast<int8_t>(sel_of(gbits[i][1]));
+                in.s1 = static_cast<int8_t>(sel_of(gbits[i][0]));
                 std::vector<cd> f;
                 for (int k = 0; k < 4; k++) {
                     f.push_back(p.U.mat[5 * k]);

@@ -19,6 +19,10 @@
 //             (controlled gates with the control on a lane, diagonal gates
 //             across a register and a lane bit);
 //   RT_LANE   diagonal on lane positions only: one factor per lane.
+// A selector s0 / s1 of 64 + b names a global index bit b outside the tile
+// instead of a lane: its value is fixed for the whole tile, so an operation
+// whose non-mixing wires (a control, a diagonal factor) lie outside the tile
+// needs no tile bit for them.
 //
 // Operations write out of place. The NS states (phi and NS - 1 observable
 // states) live in NS + 1 register buffers: each operation writes state s into
@@ -50,6 +54,13 @@ template <typename T> using CT = const __attribute__((address_space(4))) T;
 template <typename T> constexpr int rt_r() { return sizeof(T) == 8 ? 3 : 4; }
 
 __device__ __forceinline__ bool lbit(int lane, int p) { return (lane >> p) & 1; }
+
+// Block selector of RT_SEL / RT_LANE: a lane position (0-5), or 64 + a
+// global index bit outside the tile, whose value is the same for the whole
+// tile (tbase: the tile's first amplitude index).
+__device__ __forceinline__ bool selbit(int lane, uint64_t tbase, int s) {
+    return s < 64 ? lbit(lane, s) : ((tbase >> (s - 64)) & 1);
+}
 
 // Buffer of state s before (in) and after (out) instruction number `ROT`
 // (mod NS + 1) of the rotation.
@@ -397,7 +408,7 @@ __device__ __forceinline__ void remap_all(Cx<T> (&B)[NS + 1][A], int q, int p) {
 
 template <typename T, int NS, int A, int R, int ROT>
 __device__ __forceinline__ void step(Cx<T> (&B)[NS + 1][A], CT<unsigned char> *prog, int pc,
-                                     double *lacc, int lane) {
+                                     double *lacc, int lane, uint64_t tbase) {
     using RT = Rot<NS, ROT>;
     const auto *words = reinterpret_cast<CT<int32_t> *>(prog) + 8 * pc;
     int32_t w[8];
@@ -499,8 +510,8 @@ __device__ __forceinline__ void step(Cx<T> (&B)[NS + 1][A], CT<unsigned char> *p
         });
     } else if (code == RT_SEL) {
         const int nb = s1 >= 0 ? 4 : 2;
-        const bool c0 = lbit(lane, s0);
-        const bool c1 = s1 >= 0 ? lbit(lane, s1) : false;
+        const bool c0 = selbit(lane, tbase, s0);
+        const bool c1 = s1 >= 0 ? selbit(lane, tbase, s1) : false;
         dispatch_q<R>(a, [&](auto Qc) {
             constexpr int Q = decltype(Qc)::value;
             if (slot >= 0) {
@@ -554,8 +565,8 @@ __device__ __forceinline__ void step(Cx<T> (&B)[NS + 1][A], CT<unsigned char> *p
     } else {
         // RT_LANE: G phi = g phi on this lane, so the products scale by g.
         const int nb = s1 >= 0 ? 4 : 2;
-        const bool c0 = lbit(lane, s0);
-        const bool c1 = s1 >= 0 ? lbit(lane, s1) : false;
+        const bool c0 = selbit(lane, tbase, s0);
+        const bool c1 = s1 >= 0 ? selbit(lane, tbase, s1) : false;
         if (slot >= 0) {
             T g[1];
             select_block<T, 1>(g, G, nb, c0, c1);
@@ -578,12 +589,13 @@ __device__ __forceinline__ void step(Cx<T> (&B)[NS + 1][A], CT<unsigned char> *p
 // the rotation after the last instruction.
 template <typename T, int NS, int A, int R, int... ROTS>
 __device__ __forceinline__ int run_program(Cx<T> (&B)[NS + 1][A], CT<unsigned char> *prog, int ninstr,
+                                           uint64_t tbase,
                                            double *lacc, int lane,
                                            std::integer_sequence<int, ROTS...>) {
     int pc = 0;
     for (;;) {
         int end = 0;
-        const bool more = ((pc < ninstr ? (step<T, NS, A, R, ROTS>(B, prog, pc, lacc, lane), ++pc, true)
+        const bool more = ((pc < ninstr ? (step<T, NS, A, R, ROTS>(B, prog, pc, lacc, lane, tbase), ++pc, true)
                                         : (end = ROTS, false)) &&
                            ...);
         if (!more) {
@@ -682,7 +694,7 @@ __global__ void __launch_bounds__(64)
                 B[s][k] = ldo(sp[s], static_cast<Off>((base + roff(L.reg_in, k)) * sizeof(Cx<T>)) + vin);
             }
         }
-        const int rot = run_program<T, NS, A, R>(B, prog, L.ninstr, lacc, lane,
+        const int rot = run_program<T, NS, A, R>(B, prog, L.ninstr, base, lacc, lane,
                                                  std::make_integer_sequence<int, NS + 1>{});
         auto store = [&](auto RC) {
             using RT = Rot<NS, decltype(RC)::value>;
