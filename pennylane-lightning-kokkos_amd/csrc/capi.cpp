@@ -1514,6 +1514,7 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
     }
     L.ninstr = static_cast<int>(b.ins.size());
     L.nslots = slot;
+    L.prog_bytes = static_cast<int>(b.ins.size() * sizeof(RtInstr) + b.pay.size());
     out.off = b.finish();
     return out;
 }
