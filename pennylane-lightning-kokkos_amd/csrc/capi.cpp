@@ -1514,7 +1514,6 @@ RtProgram build_rt_program(const Pass &ps, int n, int R, std::vector<unsigned ch
     }
     L.ninstr = static_cast<int>(b.ins.size());
     L.nslots = slot;
-    L.prog_bytes = static_cast<int>(b.ins.size() * sizeof(RtInstr) + b.pay.size());
     out.off = b.finish();
     return out;
 }
@@ -1570,48 +1569,6 @@ PlanPlace copy_image(Device &dev, const PlanImage &img, unsigned char *base) {
     pl.rt = put(img.rt.data(), img.rt.size());
     pl.mats = put(img.mats.data(), img.mats.size());
     return pl;
-}
-
-// Writes the image's tables to host memory in the layout copy_image gives
-// them on the device (image_bytes(img) bytes at host_base).
-void pack_image(const PlanImage &img, unsigned char *host_base) {
-    auto up = [](size_t x) { return (x + 255) & ~size_t{255}; };
-    size_t off = 0;
-    auto put = [&](const void *src, size_t bytes) {
-        if (bytes) {
-            std::memcpy(host_base + off, src, bytes);
-        }
-        off += up(bytes);
-    };
-    put(img.ops.data(), img.ops.size());
-    put(img.bundles.data(), img.bundles.size() * sizeof(TileBundle));
-    put(img.rt.data(), img.rt.size());
-    put(img.mats.data(), img.mats.size());
-}
-
-// Pinned host staging for table uploads that must be one copy (grow-only,
-// per device). A caller fills it, copies it to the device and synchronises
-// the stream before its entry call returns, so the next call may reuse it.
-unsigned char *host_stage(const Device &dev, size_t bytes) {
-    struct Stage {
-        unsigned char *p = nullptr;
-        size_t cap = 0;
-    };
-    static std::map<int, Stage> stages; // released with the process
-    Stage &st = stages[dev.id];
-    if (st.cap < bytes) {
-        if (st.p) {
-            PLK_HIP(hipHostFree(st.p));
-            st.p = nullptr;
-            st.cap = 0;
-        }
-        const size_t cap = std::max<size_t>(bytes, size_t{1} << 20);
-        void *p = nullptr;
-        PLK_HIP(hipHostMalloc(&p, cap, hipHostMallocDefault));
-        st.p = static_cast<unsigned char *>(p);
-        st.cap = cap;
-    }
-    return st.p;
 }
 
 // The addresses copy_image(dev, img, base) fills, without copying.
@@ -2366,28 +2323,22 @@ void adjoint_jacobian(const State &ref, const std::vector<std::shared_ptr<const 
             list.push_back(std::move(ng));
             g = list.back().get();
         }
-        // All tables of the replay in one pinned staging buffer and one copy.
-        size_t total = g->fwd_bytes + g->rev_bytes;
-        for (const auto &pi : pimg) {
-            total += (pi.bytes.size() + 255) & ~size_t{255};
-        }
-        const size_t jac_bytes = sizeof(double) * n_obs * n_tp;
-        unsigned char *hs = host_stage(dev, total + jac_bytes);
-        pack_image(img_f, hs);
-        pack_image(img_r, hs + g->fwd_bytes);
-        size_t off = g->fwd_bytes + g->rev_bytes;
+        auto *tb = static_cast<unsigned char *>(g->tables);
+        copy_image(dev, img_f, tb);
+        copy_image(dev, img_r, tb + g->fwd_bytes);
+        unsigned char *pt = tb + g->fwd_bytes + g->rev_bytes;
         for (const auto &pi : pimg) {
             if (!pi.bytes.empty()) {
-                std::memcpy(hs + off, pi.bytes.data(), pi.bytes.size());
+                PLK_HIP(hipMemcpyAsync(pt, pi.bytes.data(), pi.bytes.size(), hipMemcpyHostToDevice,
+                                       dev.stream));
             }
-            off += (pi.bytes.size() + 255) & ~size_t{255};
+            pt += (pi.bytes.size() + 255) & ~size_t{255};
         }
-        PLK_HIP(hipMemcpyAsync(g->tables, hs, total, hipMemcpyHostToDevice, dev.stream));
         PLK_HIP(hipGraphLaunch(g->exec, dev.stream));
-        PLK_HIP(hipMemcpyAsync(hs + total, g->jac, jac_bytes, hipMemcpyDeviceToHost, dev.stream));
-        // Also keeps the staging buffer untouched until its copies have run.
+        PLK_HIP(hipMemcpyAsync(jac, g->jac, sizeof(double) * n_obs * n_tp, hipMemcpyDeviceToHost,
+                               dev.stream));
+        // Also keeps the host images alive until their copies have run.
         PLK_HIP(hipStreamSynchronize(dev.stream));
-        std::memcpy(jac, hs + total, jac_bytes);
         return;
     }
 

@@ -674,27 +674,6 @@ __global__ void __launch_bounds__(64)
         return o;
     };
     const uint64_t ntiles = uint64_t{1} << (L.n - L.T);
-    {
-        // Bring the whole program into the scalar cache with independent
-        // loads (one per 64-byte line) and one wait: a wave that runs few
-        // tiles would otherwise wait for every instruction's first fetch in
-        // turn.
-        const auto *pw = reinterpret_cast<CT<int32_t> *>(prog);
-        const int nlines = (L.prog_bytes + 63) / 64;
-        for (int l0 = 0; l0 < nlines; l0 += 16) {
-            int32_t v[16];
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                v[i] = pw[16 * min(l0 + i, nlines - 1)];
-            }
-            int32_t acc = 0;
-#pragma unroll
-            for (int i = 0; i < 16; i++) {
-                acc += v[i];
-            }
-            __asm__ volatile("" ::"s"(acc));
-        }
-    }
     for (uint64_t g = b; g < ntiles; g += bpc) {
         uint64_t base = g;
         for (int q = 0; q < L.T; q++) {

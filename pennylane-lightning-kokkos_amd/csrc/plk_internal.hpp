@@ -314,7 +314,6 @@ struct RtLaunch {
     int reg_out[RT_RMAX], lane_out[RT_LANE_BITS]; // and at store
     int nfin;                          // remaps before the store (0-2):
     int fin_rq[2], fin_rp[2];          // register bit fin_rq[f] <-> lane position fin_rp[f]
-    int prog_bytes;                    // instructions + payloads of the pass program
 };
 
 // Runs one register-tile pass over phi and nc chunks of observable states
