@@ -130,3 +130,53 @@ def test_side_by_side_chunks(capi, monkeypatch, n, nj, n_obs, dtype):
     got = capi.adjoint_jacobian(s, [to_capi(capi, o) for o in obs], capi.Ops(*ops), tp, True)
     ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
     _assert_jac(ref, got, dtype)
+
+
+SELECTOR_GATES = ["CNOT", "CZ", "CY", "CRX", "CRY", "CRZ", "ControlledPhaseShift", "IsingZZ", "RZ",
+                  "PhaseShift", "MultiRZ", "SWAP"]
+
+
+def _selector_circuit(n, count, rng):
+    """Gates whose control or diagonal wires sit on the highest wires (low
+    index bits, usually inside the tile) or the lowest wires (high index
+    bits, usually outside it), mixed with RY / RX on the other end."""
+    names, params, wires = [], [], []
+    for k in range(count):
+        g = SELECTOR_GATES[int(rng.integers(len(SELECTOR_GATES)))] if k % 2 else ["RY", "RX"][k % 4 // 2]
+        lo = [int(x) for x in rng.choice(4, size=2, replace=False)]          # wires 0-3: high bits
+        hi = [n - 1 - int(x) for x in rng.choice(4, size=2, replace=False)]  # wires n-4..n-1: low bits
+        if g in ("RY", "RX"):
+            w = [hi[0] if k % 3 else lo[0]]
+        elif g in ("RZ", "PhaseShift"):
+            w = [lo[0]]
+        elif g == "SWAP":
+            w = [lo[0], hi[0]]
+        else:
+            w = [lo[0], hi[0]] if k % 3 else [hi[0], lo[0]]
+        names.append(g)
+        params.append([float(rng.uniform(0, 2 * np.pi))] if GATES[g][1] == 1 else [])
+        wires.append(w)
+    return names, params, wires, [bool(rng.integers(2)) for _ in names], [[] for _ in names]
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+@pytest.mark.parametrize("n", [16, 20])
+def test_selectors_outside_the_tile(capi, monkeypatch, n, dtype):
+    """Controls and diagonal factors on wires outside the tile select their
+    block from the tile's fixed index bits: forward application and the
+    adjoint against the oracle."""
+    monkeypatch.setenv("PLK_GRAPH", "0")
+    rng = np.random.default_rng(9800 + n)
+    ops = _selector_circuit(n, 80, rng)
+    s = capi.State(n, dtype)
+    s.apply_ops(capi.Ops(*ops))
+    ref = O.apply_ops(O.zero_state(n), O.Ops(*ops))
+    np.testing.assert_allclose(s.download(), ref, atol=1e-12 if dtype == np.complex128 else 2e-5)
+    obs = [("named", "PauliZ", [0]), ("named", "PauliX", [n - 1]), random_hamiltonian(n, 5, rng),
+           ("tensor", [("named", "PauliY", [1]), ("named", "PauliZ", [n - 2])])]
+    npar = sum(1 for p in ops[1] if p)
+    tp = list(range(npar)) if n <= 16 else list(range(npar - 16, npar))
+    got = capi.adjoint_jacobian(capi.State(n, dtype), [to_capi(capi, o) for o in obs], capi.Ops(*ops),
+                                tp, True)
+    want = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
+    _assert_jac(want, got, dtype)
