@@ -198,3 +198,39 @@ def test_excitation_workload(capi, dtype):
     got = capi.adjoint_jacobian(s, [to_capi(capi, o) for o in obs], capi.Ops(*ops), tp, True)
     ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops), tp, True)
     np.testing.assert_allclose(got, ref, **_tol(dtype))
+
+
+@pytest.mark.parametrize("dtype", [np.complex128, np.complex64])
+def test_wide_matrices_between_side_by_side_passes(capi, monkeypatch, dtype):
+    """5- and 6-wire matrices (the per-state path) between register-tile
+    passes whose observable chunks run side by side, so lambda alternates
+    between its two buffers around them; nine observables, against the
+    oracle."""
+    monkeypatch.setenv("PLK_GRAPH", "0")
+    rng = np.random.default_rng(9600)
+    n = 14
+    names, params, wires, inverses, mats = [], [], [], [], []
+    for k in range(30):
+        if k % 10 == 5:
+            m = 5 if k < 20 else 6
+            names.append("QubitUnitary")
+            params.append([])
+            wires.append([int(x) for x in rng.choice(n, size=m, replace=False)])
+            inverses.append(bool(rng.integers(2)))
+            mats.append(random_unitary(2**m, rng).reshape(-1))
+        else:
+            g = ["RX", "RY", "CNOT", "IsingXX"][k % 4]
+            names.append(g)
+            params.append([float(rng.uniform(0, 2 * np.pi))] if g != "CNOT" else [])
+            wires.append([int(x) for x in rng.choice(n, size=1 if g in ("RX", "RY") else 2,
+                                                     replace=False)])
+            inverses.append(False)
+            mats.append([])
+    ops = (names, params, wires, inverses, mats)
+    obs = _obs(n, rng, 9)
+    npar = sum(1 for p in params if p)
+    got = capi.adjoint_jacobian(capi.State(n, dtype), [to_capi(capi, o) for o in obs], capi.Ops(*ops),
+                                list(range(npar)), True)
+    ref = O.adjoint_jacobian(O.zero_state(n), [to_oracle(o) for o in obs], O.Ops(*ops),
+                             list(range(npar)), True)
+    np.testing.assert_allclose(got, ref, **_tol(dtype))
