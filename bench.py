@@ -8,10 +8,12 @@ A step is the full adjoint Jacobian as the device computes it: forward
 application of the circuit to |0...0>, the observables applied to the state,
 and the fused backward sweep. Inputs are resident in HBM before timing.
 
-Prints one JSON line. `roofline` refers to the dominant kernel (the fused
-adjoint segment): algorithmic bytes per launch / average launch duration,
-both measured with HIP events on the library's stream during the timed
-steps. `cpu_baseline` times the CPU oracle (the reference algorithm restated
+Prints one JSON line. `roofline` refers to the dominant kernel (the
+register-tile adjoint pass: one launch is one pass of the backward sweep for
+all observables, (1 + 1 + 2 n_obs) S algorithmic bytes): algorithmic bytes
+per launch / average launch duration, both measured with HIP events on the
+library's stream during the timed steps; `traffic` is the rocprofv3-measured
+HBM bytes per launch while the kernel sources match the profiled ones. `cpu_baseline` times the CPU oracle (the reference algorithm restated
 in C++/OpenMP) on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -600,7 +602,10 @@ def main():
                            "adjoint_segment": "seg_kernel (register segment, adjoint)"}.get(dom, dom),
                 "avg_launch_ms": round(avg_ms, 4), "avg_launch_bytes": avg_bytes,
                 "launches_per_step": seg["launches"] / args.steps,
-                "copy_ceiling_GBps": ceiling, "frac_of_copy_ceiling": round(achieved / ceiling, 4)}
+                "copy_ceiling_GBps": ceiling, "frac_of_copy_ceiling": round(achieved / ceiling, 4),
+                "note": ("bytes per launch are fixed by the pass (every state read and written "
+                         "once), so packing more operations into a pass lowers GB/s while it "
+                         "cuts the passes, and the bytes, of a Jacobian")}
     # Per family: algorithmic bytes (what a launch must read and write; the
     # Pauli-sum kernels count every psi[i ^ x] gather as a read, many of which
     # hit in cache) over HIP-event time. Host scopes carry no bytes.
