@@ -100,6 +100,16 @@ def serialize_ops(tape: Tape):
     return (names, params, wires, inverses, mats), uses_stateprep
 
 
+def batch_eligible(serialized, uses_stateprep: bool, observables) -> bool:
+    """Whether a serialized tape can run in the batched engine
+    (`plk_adjoint_batched`): no state preparation, no explicit matrices,
+    operations on at most two wires, exactly one observable that is a sum of
+    Pauli words."""
+    _, _, wires, _, mats = serialized
+    return (not uses_stateprep and len(observables) == 1 and _pauli_only(observables[0]) and
+            all(len(w) <= 2 for w in wires) and all(len(m) == 0 for m in mats))
+
+
 _NAMED = {
     "Identity": np.eye(2, dtype=complex),
     "PauliX": np.array([[0, 1], [1, 0]], dtype=complex),
@@ -367,10 +377,7 @@ class LightningKokkosAMD:
                       for o in tape.operations]
             ser, uses_sp = serialize_ops(Tape(mapped, []))
             names, params, wires, inverses, mats = ser
-            eligible = (not uses_sp and len(tape.observables) == 1 and
-                        _pauli_only(tape.observables[0]) and
-                        all(len(w) <= 2 for w in wires) and all(len(m) == 0 for m in mats))
-            if not eligible:
+            if not batch_eligible(ser, uses_sp, tape.observables):
                 out[k] = self.adjoint_jacobian(tape)
                 continue
             batch_ops.append(_capi.Ops(names, params, wires, inverses, mats))

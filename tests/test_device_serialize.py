@@ -30,3 +30,26 @@ def test_trainable_bookkeeping_counts_stateprep_parameter():
     assert tape.get_trainable() == [0, 1, 2]
     assert [o for o, _ in tape.all_params()] == [0, 1, 2]
     assert LightningKokkosAMD.short_name == "lightning.kokkos"
+
+
+def test_batch_eligibility():
+    """Tapes the batched engine takes: no state preparation, no explicit
+    matrix, operations on at most two wires, one Pauli-sum observable."""
+    from pennylane_lightning_kokkos_amd.device import batch_eligible
+
+    ham = ("ham", [0.5, -1.0], [("named", "PauliZ", [0]),
+                                ("tensor", [("named", "PauliX", [1]), ("named", "PauliY", [2])])])
+    base = [Op("Rot", [0], [0.1, 0.2, 0.3]), Op("CNOT", [0, 1]), Op("IsingXX", [1, 2], [0.4])]
+
+    def ok(ops, obs):
+        ser, sp = serialize_ops(Tape(ops, obs))
+        return batch_eligible(ser, sp, obs)
+
+    assert ok(base, [ham])
+    assert ok(base, [("named", "PauliZ", [2])])
+    assert not ok(base, [ham, ("named", "PauliZ", [2])])                  # two observables
+    assert not ok(base, [("herm", np.eye(2), [0])])                       # not a Pauli sum
+    assert not ok(base + [Op("Toffoli", [0, 1, 2])], [ham])               # three wires
+    assert not ok([Op("BasisState", [0], [np.array([1])])] + base, [ham])  # state preparation
+    u = np.array([[0, 1], [1, 0]], dtype=np.complex128)
+    assert not ok(base + [Op("QubitUnitary", [1], [], False, u)], [ham])  # explicit matrix
